@@ -1,0 +1,176 @@
+"""Dense MASDA throughput on one MI355X: 848x480 pair, 128 disparities, 8 BP
+iterations (the C2 workload), through the C ABI with inputs resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints ONE JSON line. `value` = Mpx*disp/s of the whole job (W*H*D per frame
+over the timed region). `roofline` is for the dominant kernel, timed live with
+hipEvents recorded between the stage launches on the launching stream;
+`cpu_baseline` times the reference's own CPU tool (oracle/_ref/de_dense,
+built from the reference sources) on this host, or the oracle port if that is
+absent, on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W, H, D = 848, 480, 128
+CFG = dict(dmax=128, iters=8, sigma_s=20.0, sigma_r=0.0392)
+REF_FLAGS = ["--dmax", "128", "--iters", "8", "--sigma-s", "20", "--sigma-r", "0.0392"]
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(w: int, h: int, d: int) -> dict:
+    """Algorithmic HBM bytes per frame and per stage kernel.
+
+    Frame: 2WH (pair) + 4WHD (int16 volume written once, read once) + 48WH
+    (24 B/px of top-2 candidates written and read) + 8WH (disparity + margin).
+    Stages: the row kernel reads the pair and writes the volume once (2WH +
+    2WHD); the column kernel reads the volume once and writes 16 B/px candidate
+    records (2WHD + 16WH); the solver reads the records and writes disparity and
+    margin (16WH + 8WH); prep reads the pair and writes its 28 B/px census and
+    coefficient records (2WH + 28WH)."""
+    wh = w * h
+    return {
+        "frame": 2 * wh + 4 * wh * d + 48 * wh + 8 * wh,
+        "prep": 2 * wh + 28 * wh,
+        "row": 2 * wh + 2 * wh * d,
+        "col": 2 * wh * d + 16 * wh,
+        "solve": 16 * wh + 8 * wh,
+    }
+
+
+def cpu_baseline(L, R, budget_s: float = 12.0):
+    """Reference CPU tool at --threads 1 on the same pair, repeated to ~budget_s."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "de_dense")
+    with tempfile.TemporaryDirectory() as td:
+        lp, rp = os.path.join(td, "l.y8"), os.path.join(td, "r.y8")
+        L.tofile(lp)
+        R.tofile(rp)
+        if os.access(ref, os.X_OK):
+            times = []
+            t_end = time.time() + budget_s
+            while time.time() < t_end or not times:
+                out = subprocess.run([ref, lp, rp, str(W), str(H), "--threads", "1", *REF_FLAGS],
+                                     capture_output=True, text=True, check=True).stdout
+                m = re.findall(r"total ([0-9.]+) ms", out)
+                times.append(float(m[-1]))
+            ms = sum(times) / len(times)
+            return {"value": round(W * H * D / (ms * 1e-3) / 1e6, 2), "unit": "Mpx*disp/s",
+                    "cores": 1, "kind": "reference",
+                    "sample": f"{len(times)} frames of 848x480x128 iters 8, de_dense --threads 1 "
+                              f"(its own census+cost+solve timer), mean {ms:.1f} ms/frame"}
+    from oracle import binding as orc
+    c = orc.cfg(**CFG)
+    times = []
+    t_end = time.time() + budget_s
+    while time.time() < t_end or not times:
+        t0 = time.perf_counter()
+        orc.dense_run(L, R, c)
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms = sum(times) / len(times)
+    return {"value": round(W * H * D / (ms * 1e-3) / 1e6, 2), "unit": "Mpx*disp/s",
+            "cores": 1, "kind": "port",
+            "sample": f"{len(times)} frames of 848x480x128 iters 8 through the oracle, "
+                      f"mean {ms:.1f} ms/frame"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the dense frame shards by disparity range or by stream; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from doubleeye_amd import synth
+    from doubleeye_amd.dense import Cfg, Context
+
+    L, R, _ = synth.slanted_pair(W, H, seed=2)
+    dev = torch.device("cuda:0")
+    dL = torch.from_numpy(L).to(dev)
+    dR = torch.from_numpy(R).to(dev)
+    disp = torch.empty((H, W), dtype=torch.float32, device=dev)
+    margin = torch.empty((H, W), dtype=torch.float32, device=dev)
+    ctx = Context(0)
+    cfg = Cfg(**CFG)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def step():
+        ctx.run_dense_dev(dL.data_ptr(), dR.data_ptr(), W, H, cfg, disp.data_ptr(),
+                          margin.data_ptr(), 0, stream)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / a.steps
+    value = W * H * D / (ms * 1e-3) / 1e6
+
+    # per-stage kernel durations, hipEvents between launches on this stream
+    ctx.set_profiling(True)
+    for _ in range(max(20, a.steps // 4)):
+        step()
+    torch.cuda.synchronize()
+    stage_ms, runs = ctx.stage_ms()
+    ctx.set_profiling(False)
+    names = ["prep", "row", "col", "solve"]
+    per = {n: stage_ms[i] / max(1, runs) for i, n in enumerate(names)}
+    dom = max(per, key=per.get)
+    ab = algorithmic_bytes(W, H, D)
+    achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
+    frame_gbs = ab["frame"] / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "dense Mpx*disp/s at 848x480x128 (C2, 8 BP iterations)",
+        "value": round(value, 1),
+        "unit": "Mpx*disp/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 4),
+        "frames_per_s": round(1e3 / ms, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i16 volume / f32 messages",
+        "data": "synthetic (seeded slanted-plane pair, doubleeye_amd.synth.slanted_pair seed 2)",
+        "config": {"workload": "dense MASDA 848x480, D=128 (d=1..128), census 7x7, "
+                               "sigma_s 20, sigma_r 10/255, 8 BP iterations, top-2",
+                   "W": W, "H": H, "D": D, "iters": 8},
+        "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": round(per[dom], 4),
+                     "stage_ms": {n: round(v, 4) for n, v in per.items()},
+                     "frame_algorithmic_bytes": ab["frame"],
+                     "frame_achieved_gbs": round(frame_gbs, 1),
+                     "frame_frac": round(frame_gbs / HBM_PEAK_GBS, 4)},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(np.ascontiguousarray(L), np.ascontiguousarray(R),
+                                           a.cpu_budget)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,88 @@
+"""ctypes loader for libdoubleeye_hip.so (the C ABI in include/doubleeye_hip.h).
+
+The library is the product path: if it is missing or cannot be loaded this
+module raises; there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdoubleeye_hip.so")
+
+DE_OK = 0
+DE_ERR_ARG = -1
+DE_ERR_UNSUPPORTED = -2
+DE_ERR_HIP = -3
+DE_ERR_NOMEM = -4
+
+
+class DeError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"doubleeye_hip error {code}: {msg}")
+        self.code = code
+
+
+class DeCfgC(C.Structure):
+    """de_cfg (include/doubleeye_hip.h)."""
+    _fields_ = [("dmin", C.c_int), ("dmax", C.c_int), ("iters", C.c_int),
+                ("census_hw", C.c_int), ("census_hh", C.c_int),
+                ("sigma_s", C.c_float), ("sigma_r", C.c_float),
+                ("ad", C.c_float), ("ad_trunc", C.c_int),
+                ("lambda_", C.c_float), ("gamma", C.c_float),
+                ("damping", C.c_float), ("min_margin", C.c_float),
+                ("min_ratio", C.c_float),
+                ("subpixel", C.c_int), ("fit_eq", C.c_int), ("lrc", C.c_int)]
+
+
+# (name, restype, argtypes) for every exported entry point
+P = C.c_void_p
+I = C.c_int
+F = C.c_float
+SIGNATURES = {
+    "de_default_cfg": (None, [C.POINTER(DeCfgC)]),
+    "de_ctx_create": (I, [I, C.POINTER(P)]),
+    "de_ctx_destroy": (None, [P]),
+    "de_ctx_sync": (I, [P]),
+    "de_ctx_stream": (P, [P]),
+    "de_last_error": (C.c_char_p, []),
+    "de_census": (I, [P, P, I, I, I, I, P]),
+    "de_census_dev": (I, [P, P, I, I, I, I, P, P]),
+    "de_dense_run": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, P, P]),
+    "de_dense_run_dev": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, P, P, P]),
+    "de_dense_volume": (I, [P, P, P, I, I, C.POINTER(DeCfgC), I, I, P]),
+    "de_dense_candidates": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P]),
+    "de_ctx_set_profiling": (I, [P, I]),
+    "de_ctx_stage_ms": (I, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"{LIB_PATH} is missing: build it with `python -m doubleeye_amd.build` "
+                    "(there is no CPU fallback)")
+            L = C.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(L, name, None)
+                if fn is None:
+                    continue
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != DE_OK:
+        msg = lib().de_last_error()
+        raise DeError(rc, msg.decode() if msg else "")

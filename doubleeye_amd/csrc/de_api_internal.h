@@ -1,0 +1,90 @@
+// Host-side internals shared by the C-ABI translation units.
+#ifndef DOUBLEEYE_DE_API_INTERNAL_H
+#define DOUBLEEYE_DE_API_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/doubleeye_hip.h"
+#include "de_batch.h"
+#include "de_common.h"
+
+namespace de {
+
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+int validate(const de_cfg& c, int W, int H);
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t size = 0;
+  int ensure(size_t bytes);
+  void release();
+};
+
+// One frame's device scratch.
+struct Scratch {
+  uint32_t* lrec;
+  uint64_t* cr;
+  uint32_t* ayt;
+  uint32_t* cand;
+  uint16_t* vol;
+};
+size_t scratch_bytes(int W, int H, int D);
+Scratch carve_scratch(void* base, int W, int H, int D);
+
+struct Ctx {
+  static constexpr int kSlots = 16;
+  struct Slot {
+    void* dev = nullptr;
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool used = false;
+  };
+  int device = -1;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  DevBuf scratch, rf_buf, sc_buf, io_in, io_out;
+  Slot slots[kSlots];
+  int next_slot = 0;
+  bool store_vol = false;
+  // optional per-stage timing: hipEvents recorded between the stage launches
+  struct EvSet { hipEvent_t e[kNumStages + 1]; };
+  bool profiling = false;
+  std::vector<EvSet> ev_pending, ev_pool;
+  double stage_ms[kNumStages] = {0, 0, 0, 0};
+  int stage_runs = 0;
+  int take_evset(EvSet* out);
+  int drain_events();
+  // cached table key
+  bool have_tables = false;
+  float tk_sigma_s = 0, tk_sigma_r = 0, tk_ad = 0;
+  int tk_nbits = 0, tk_T = 0;
+
+  ~Ctx();
+  int tables_for(const de_cfg& c, const uint16_t** rf, const uint16_t** sc, int* shift);
+  int acquire_slot(size_t bytes, Slot** out);
+  int frame_desc(const de_cfg& c, int W, int H, const uint8_t* L, const uint8_t* R,
+                 float* disp, float* margin, float* cand6, uint8_t* conf, int16_t* q4,
+                 const Scratch& sp, FrameDesc* fd);
+  int run_frames(const FrameDesc* fds, int n, hipStream_t s, int stage_mask);
+  int dense_dev(const uint8_t* dL, const uint8_t* dR, int W, int H, const de_cfg& c,
+                float* disp, float* margin, float* cand6, hipStream_t s, int stage_mask);
+};
+
+// kernel launchers
+size_t row_lds_bytes(int W, int nbits, int sc_shift);
+size_t solve_lds_bytes(int W);
+hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_t s);
+hipError_t launch_row(const BatchView& b, int items, size_t lds, hipStream_t s);
+hipError_t launch_col(const BatchView& b, int items, int gmax, hipStream_t s);
+hipError_t launch_solve(const BatchView& b, int items, size_t lds, hipStream_t s);
+hipError_t launch_census_plane(const uint8_t* img, int W, int H, int hw, int hh,
+                               uint64_t* out, hipStream_t s);
+
+}  // namespace de
+
+#endif

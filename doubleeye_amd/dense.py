@@ -1,0 +1,183 @@
+"""Dense MASDA on the MI355X through the C ABI.
+
+Mirrors the reference's dense entry points:
+
+* ``Cfg`` -- the default-path fields of ``doubleeye::Cfg``
+  (core/include/doubleeye/dense_solve.hpp:19-97), same names and defaults.
+* ``run_dense(L, R, cfg)`` -- ``run_dense`` (core/tools/de_dense.cpp:738) on its
+  default path, i.e. what ``de_dense --threads 1`` writes with ``--out`` and
+  ``--out-conf``.
+* ``census(img, hw, hh)`` -- ``census_transform`` (core/src/preproc.cpp:110).
+
+Errors are raised as ``DeError`` (the reference exits 1/2 with a message).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, fields
+
+import numpy as np
+
+from ._lib import DeCfgC, check, lib
+
+__all__ = ["Cfg", "Context", "run_dense", "census", "default_context"]
+
+
+@dataclass
+class Cfg:
+    dmin: int = 1
+    dmax: int = 60
+    iters: int = 0
+    census_hw: int = 3
+    census_hh: int = 3
+    sigma_s: float = 8.0
+    sigma_r: float = 0.2
+    ad: float = 0.15
+    ad_trunc: int = 10
+    lam: float = -0.1          # lambda (a Python keyword)
+    gamma: float = -0.1
+    damping: float = 0.4
+    min_margin: float = 0.0
+    min_ratio: float = 0.0
+    subpixel: bool = True
+    fit_eq: bool = True
+    lrc: bool = False
+
+    @property
+    def D(self) -> int:
+        return self.dmax - self.dmin + 1
+
+    def to_c(self) -> DeCfgC:
+        c = DeCfgC()
+        for f in fields(self):
+            name = "lambda_" if f.name == "lam" else f.name
+            v = getattr(self, f.name)
+            setattr(c, name, int(v) if isinstance(v, bool) else v)
+        return c
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """A device context: stream + resident scratch (de_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().de_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            lib().de_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return int(lib().de_ctx_stream(self.h) or 0)
+
+    def sync(self):
+        check(lib().de_ctx_sync(self.h))
+
+    # --- host-pointer API ---
+    def census(self, img: np.ndarray, hw: int = 3, hh: int = 3) -> np.ndarray:
+        img = np.ascontiguousarray(img, np.uint8)
+        out = np.empty(img.shape, np.uint64)
+        check(lib().de_census(self.h, _p(img), img.shape[1], img.shape[0], hw, hh, _p(out)))
+        return out
+
+    def run_dense(self, L: np.ndarray, R: np.ndarray, cfg: Cfg | None = None,
+                  want_cand: bool = False):
+        cfg = cfg or Cfg()
+        L = np.ascontiguousarray(L, np.uint8)
+        R = np.ascontiguousarray(R, np.uint8)
+        if L.shape != R.shape or L.ndim != 2:
+            raise ValueError(f"left {L.shape} and right {R.shape} must be equal 2-D images")
+        H, W = L.shape
+        disp = np.empty((H, W), np.float32)
+        margin = np.empty((H, W), np.float32)
+        cand = np.empty((6, H, W), np.float32) if want_cand else None
+        c = cfg.to_c()
+        check(lib().de_dense_run(self.h, _p(L), _p(R), W, H, C.byref(c), _p(disp), _p(margin),
+                                 _p(cand) if cand is not None else None))
+        return disp, margin, cand
+
+    def volume(self, L, R, cfg: Cfg, k0: int, k1: int) -> np.ndarray:
+        L = np.ascontiguousarray(L, np.uint8)
+        R = np.ascontiguousarray(R, np.uint8)
+        H, W = L.shape
+        out = np.empty((k1 - k0, H, W), np.int16)
+        c = cfg.to_c()
+        check(lib().de_dense_volume(self.h, _p(L), _p(R), W, H, C.byref(c), k0, k1, _p(out)))
+        return out
+
+    def candidates(self, L, R, cfg: Cfg) -> np.ndarray:
+        L = np.ascontiguousarray(L, np.uint8)
+        R = np.ascontiguousarray(R, np.uint8)
+        H, W = L.shape
+        out = np.empty((H, W, 4), np.uint32)
+        c = cfg.to_c()
+        check(lib().de_dense_candidates(self.h, _p(L), _p(R), W, H, C.byref(c), _p(out)))
+        return out
+
+    # --- device-pointer API (ints are device addresses, e.g. tensor.data_ptr()) ---
+    def run_dense_dev(self, dL: int, dR: int, W: int, H: int, cfg: Cfg, d_disp: int,
+                      d_margin: int, d_cand6: int = 0, stream: int = 0):
+        c = cfg.to_c()
+        check(lib().de_dense_run_dev(self.h, C.c_void_p(dL), C.c_void_p(dR), W, H, C.byref(c),
+                                     C.c_void_p(d_disp), C.c_void_p(d_margin),
+                                     C.c_void_p(d_cand6) if d_cand6 else None,
+                                     C.c_void_p(stream) if stream else None))
+
+    def census_dev(self, d_img: int, W: int, H: int, hw: int, hh: int, d_out: int,
+                   stream: int = 0):
+        check(lib().de_census_dev(self.h, C.c_void_p(d_img), W, H, hw, hh, C.c_void_p(d_out),
+                                  C.c_void_p(stream) if stream else None))
+
+    def set_profiling(self, on: bool):
+        check(lib().de_ctx_set_profiling(self.h, int(on)))
+
+    def stage_ms(self):
+        ms = (C.c_double * 4)()
+        runs = C.c_int()
+        check(lib().de_ctx_stage_ms(self.h, ms, C.byref(runs)))
+        return list(ms), runs.value
+
+
+_default: Context | None = None
+
+
+def default_context() -> Context:
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+def run_dense(L, R, cfg: Cfg | None = None, want_cand: bool = False):
+    """(disp, margin, cand6 or None) as de_dense --threads 1 writes them."""
+    return default_context().run_dense(L, R, cfg, want_cand)
+
+
+def census(img, hw: int = 3, hh: int = 3) -> np.ndarray:
+    return default_context().census(img, hw, hh)
+
+
+def _close_default():
+    global _default
+    if _default is not None:
+        _default.close()
+        _default = None
+
+
+import atexit as _atexit  # noqa: E402
+
+_atexit.register(_close_default)

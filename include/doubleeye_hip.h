@@ -1,0 +1,103 @@
+/* C ABI of the MI355X (gfx950) DoubleEye stereo-matching hot path.
+ *
+ * Plain C: POD structs, pointers and sizes, int status returns (0 = ok), no
+ * exceptions across the boundary, caller-owned buffers. Functions named *_dev
+ * take DEVICE pointers and an optional hipStream_t (as void*, NULL = the
+ * context's stream) and return without synchronising; the others take HOST
+ * pointers and return when the results are in host memory.
+ *
+ * Every entry point cites the reference interface it stands in for (paths
+ * relative to the reference repository root). Library: libdoubleeye_hip.so.
+ */
+#ifndef DOUBLEEYE_HIP_H
+#define DOUBLEEYE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DE_OK 0
+#define DE_ERR_ARG -1      /* invalid argument (sizes, null pointers, ranges) */
+#define DE_ERR_UNSUPPORTED -2
+#define DE_ERR_HIP -3      /* HIP runtime error; see de_last_error() */
+#define DE_ERR_NOMEM -4
+
+/* Dense configuration: the default-path fields of doubleeye::Cfg
+ * (core/include/doubleeye/dense_solve.hpp:19-97) plus the census window.
+ * census_hw/hh = 3/3 is the reference's 7x7 (48 bits); 4/3 is the 9x7 (62-bit)
+ * extension. Disparities searched are d = dmin + k, k in [0, dmax - dmin]. */
+typedef struct de_cfg {
+  int dmin, dmax, iters;
+  int census_hw, census_hh;
+  float sigma_s, sigma_r;
+  float ad;
+  int ad_trunc;
+  float lambda, gamma, damping, min_margin, min_ratio;
+  int subpixel, fit_eq, lrc;
+} de_cfg;
+
+typedef struct de_ctx de_ctx;
+
+/* Reference defaults (dense_solve.hpp:34-97): dmin 1, dmax 60, iters 0,
+ * sigma_s 8, sigma_r 0.2, ad 0.15, ad_trunc 10, lambda = gamma = -0.1,
+ * damping 0.4, subpixel on, equiangular fit. */
+void de_default_cfg(de_cfg* cfg);
+
+/* Context: device, stream, resident scratch (grown on demand, reused). One
+ * context may be used from several threads; calls are serialised internally. */
+int de_ctx_create(int device, de_ctx** out);
+void de_ctx_destroy(de_ctx* ctx);
+int de_ctx_sync(de_ctx* ctx);
+/* The context's stream (hipStream_t) as void*. */
+void* de_ctx_stream(de_ctx* ctx);
+/* Per-stage timing with hipEvents recorded between the stage launches on the
+ * launching stream (prep, row, column, solve). Enabling resets the sums;
+ * de_ctx_stage_ms returns the summed milliseconds per stage and the number of
+ * launch sequences measured. No counterpart in the reference, which times its
+ * kernels with cudaEvents inside the tool (core/tools/de_dense_cuda.cu:552-565). */
+int de_ctx_set_profiling(de_ctx* ctx, int on);
+int de_ctx_stage_ms(de_ctx* ctx, double* ms4, int* runs);
+/* Message of the last failure on this thread ("" if none). */
+const char* de_last_error(void);
+
+/* Census transform of one image, bit b in dy-major/dx-minor order with the
+ * centre skipped, set iff neighbour < centre, 0 where the window is cut.
+ * Replaces census_transform (core/include/doubleeye/preproc.hpp:92,
+ * core/src/preproc.cpp:110-138) and census_image (core/tools/de_dense.cpp:186).
+ * hw/hh in {1/1, 2/2, 3/3, 4/3}. */
+int de_census(de_ctx* ctx, const uint8_t* img, int W, int H, int hw, int hh,
+              uint64_t* out);
+int de_census_dev(de_ctx* ctx, const uint8_t* d_img, int W, int H, int hw, int hh,
+                  uint64_t* d_out, void* stream);
+
+/* The dense pipeline of run_dense (core/tools/de_dense.cpp:738-1608) on its
+ * default path (recursive filter, blockwise top-2, K = 2 row solve), equal to
+ * `de_dense --threads 1`. Outputs: disp W*H float (NaN = no answer),
+ * margin W*H float, cand6 (nullable) 6 planes s1, s2, d1, d2, lrc, lrc2 as
+ * written by --out-conf (-1e30 = missing candidate). */
+int de_dense_run(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+                 const de_cfg* cfg, float* disp, float* margin, float* cand6);
+int de_dense_run_dev(de_ctx* ctx, const uint8_t* d_L, const uint8_t* d_R, int W,
+                     int H, const de_cfg* cfg, float* d_disp, float* d_margin,
+                     float* d_cand6, void* stream);
+
+/* Intermediate for parity tests: the aggregated int16 Q14 volume for
+ * disparities [k0, k1), layout [k][y][x] (the plane rf_filter_i16 leaves,
+ * core/tools/de_dense.cpp:485-508). Host pointers. */
+int de_dense_volume(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+                    const de_cfg* cfg, int k0, int k1, int16_t* out);
+
+/* Intermediate for parity tests: per-pixel candidate records after the fused
+ * vertical filter + top-2 (4 x uint32: key0, key1, neighbour pair, 0), host
+ * pointer W*H*4 words. key = ((v + 16385) << 8) | (255 - k), 0 = none. */
+int de_dense_candidates(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W,
+                        int H, const de_cfg* cfg, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DOUBLEEYE_HIP_H */

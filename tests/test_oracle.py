@@ -1,0 +1,188 @@
+"""Pins the CPU oracle (oracle/cpu_ref.cpp) against the reference's own outputs
+and known answers, and against closed forms. CPU only.
+
+Sources of truth:
+  * tests/golden/scenes.npz   -- the reference's six IR scene pairs and their
+    meta.txt coverage_pct known answers (article/scenes/*/meta.txt), plus sha256
+    of `de_dense --dmax 64 --threads 1 --out`.
+  * tests/golden/dense_c1.npz -- `de_dense --threads 1` outputs on a seeded
+    424x240 pair for six flag sets.
+  * tests/golden/volume_f32.npz -- `de_dense --dump-vol` (FP32 volume).
+  * core/tests/test_confidence.cpp:52-56, 117-129 golden values.
+  * core/tests/test_preproc.cpp:50-93 census invariants.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, flags_to_kwargs
+
+SCENES = ["night_on", "night_off", "evening_on", "evening_off", "morning_on", "morning_off"]
+
+
+@pytest.fixture(scope="module")
+def scenes():
+    return np.load(os.path.join(GOLDEN, "scenes.npz"))
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return np.load(os.path.join(GOLDEN, "dense_c1.npz"))
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("scene", SCENES)
+def test_scene_known_answer(oracle, scenes, scene):
+    L, R = scenes[f"{scene}_L"], scenes[f"{scene}_R"]
+    d, _, _, _ = oracle.dense_run(L, R, oracle.cfg(dmax=64))
+    cov = round(100.0 * np.isfinite(d).mean(), 1)
+    assert cov == float(scenes[f"{scene}_coverage_pct"])
+    assert sha(d) == str(scenes[f"{scene}_disp_sha256"]), "differs from de_dense --threads 1"
+
+
+@pytest.mark.parametrize("name", ["c1", "default48", "lrc_it2", "nosub_margin",
+                                  "parabola_ratio", "d96_it8"])
+def test_c1_flags_bit_exact(oracle, c1, name):
+    kw = flags_to_kwargs(str(c1[f"{name}_flags"]))
+    d, _, cand, _ = oracle.dense_run(c1["L"], c1["R"], oracle.cfg(**kw), want_cand=True)
+    assert sha(d) == str(c1[f"{name}_disp_sha256"])
+    assert sha(cand) == str(c1[f"{name}_cand_sha256"])
+    if name == "c1":
+        np.testing.assert_array_equal(d.view(np.uint32), c1["c1_disp"].view(np.uint32))
+
+
+def test_volume_f32_matches_dump(oracle):
+    z = np.load(os.path.join(GOLDEN, "volume_f32.npz"))
+    kw = flags_to_kwargs(str(z["flags"]))
+    vol = oracle.volume_f32(z["L"], z["R"], oracle.cfg(**kw))
+    ref = z["vol"]
+    none = ref <= -1e29
+    assert np.array_equal(none, vol <= -1e29)
+    rel = np.abs(vol[~none] - ref[~none]) / np.maximum(np.abs(ref[~none]), 1e-6)
+    assert rel.max() <= 1e-5          # FP32 tolerance of the float path
+
+
+def test_census_bits_and_invariants(oracle):
+    rng = np.random.default_rng(7)
+    img = rng.integers(0, 256, (40, 53), dtype=np.uint8)
+    # bit count: on a strictly increasing ramp every neighbour before the
+    # centre in scan order is smaller -> exactly the low nbits/2 bits are set
+    ramp = np.arange(40 * 53, dtype=np.int64).reshape(40, 53)
+    ramp = (ramp * 255 // ramp.max()).astype(np.uint8)
+    for hw, hh, nbits in ((3, 3, 48), (4, 3, 62)):
+        c = oracle.census(img, hw, hh)
+        assert int(c.max()) < (1 << nbits)
+        cr = oracle.census(np.arange(9 * 9, dtype=np.uint8).reshape(9, 9), hw, hh)
+        assert int(cr[4, 4]) == (1 << (nbits // 2)) - 1
+        # uniform image -> all zero (test_preproc.cpp:59-63)
+        assert not oracle.census(np.full((20, 20), 77, np.uint8), hw, hh).any()
+        # monotonic invariance (test_preproc.cpp:68-93): a strictly increasing
+        # intensity map leaves every descriptor unchanged
+        small = (img // 2).astype(np.uint8)                      # 0..127
+        lut = np.sort(np.random.default_rng(hw).choice(256, 128, replace=False)).astype(np.uint8)
+        assert np.array_equal(oracle.census(small, hw, hh), oracle.census(lut[small], hw, hh))
+        # brute force at a few pixels
+        for (y, x) in ((hh, hw), (20, 25), (40 - hh - 1, 53 - hw - 1)):
+            v, b = 0, 0
+            for dy in range(-hh, hh + 1):
+                for dx in range(-hw, hw + 1):
+                    if dx == 0 and dy == 0:
+                        continue
+                    if img[y + dy, x + dx] < img[y, x]:
+                        v |= 1 << b
+                    b += 1
+            assert int(c[y, x]) == v
+        # border is zero
+        assert not c[:hh].any() and not c[-hh:].any()
+        assert not c[:, :hw].any() and not c[:, -hw:].any()
+    assert ramp.max() == 255
+
+
+def test_confidence_goldens(oracle):
+    lam = -0.1
+    for s1, s2, want in ((0.9, 0.3, 1.000000), (0.5, 0.49, 0.700536),
+                         (0.682, 0.537, 0.975261), (0.2, -0.5, 0.961586),
+                         (0.3, -1e30, 0.986346)):
+        assert abs(oracle.confidence(s1, s2, lam) - want) <= 1e-5
+    assert oracle.confidence(-1e30, -1e30, lam) == 0.0
+    assert oracle.confidence_u8(0.9, 0.3, lam) == 255
+    assert oracle.confidence_u8(0.5, 0.49, lam) == 179
+    assert abs(oracle.confidence(0.9, 0.3, lam, 9.0) - 0.35) <= 1e-6
+    assert oracle.confidence(0.9, 0.3, lam, float("nan")) == oracle.confidence(0.9, 0.3, lam, 0.0)
+
+
+def test_filter_step_offset_identity():
+    """The device form of one recursive-filter step (offset u16, two
+    multiply-adds, high-half read) equals the reference step
+    f = c + ((a*(f-c) + 2^14) >> 15) (core/tools/de_dense.cpp:411-430)."""
+    rng = np.random.default_rng(0)
+    n = 3_000_000
+    a = np.concatenate([rng.integers(0, 32768, n), [0, 0, 32767, 32767, 1, 16384]])
+    c = np.concatenate([rng.integers(-16384, 16385, n), [-16384, 16384] * 3])
+    f = np.concatenate([rng.integers(-16384, 16385, n), [16384, -16384] * 3])
+    ref = c + ((a * (f - c) + (1 << 14)) >> 15)
+    cp, fp = c + 16384, f + 16384
+    acc = (65536 - 2 * a) * cp + 32768          # v_mad_u32_u24
+    t2 = (2 * a) * fp + acc                      # v_mad_u32_u16, op_sel hi
+    assert t2.max() < 2 ** 32 and acc.max() < 2 ** 32
+    assert np.array_equal((t2 >> 16) - 16384, ref)
+    assert ref.min() >= -16384 and ref.max() <= 16384
+
+
+def _seq_top2(vals, kmax):
+    b0 = b1 = -32768
+    i0 = i1 = -1
+    for k in range(kmax + 1):
+        v = vals[k]
+        if v <= b1:
+            continue
+        if v > b0:
+            b1, i1, b0, i0 = b0, i0, v, k
+        else:
+            b1, i1 = v, k
+    return b0, i0, b1, i1
+
+
+def test_packed_top2_butterfly_semantics():
+    """The column pass's wave64 reduction: lane-local insert over G groups, then
+    a 6-round xor butterfly on packed keys ((v'+1) << 8 | 255-k). Simulated
+    here against the sequential ascending-k strictly-greater scan that defines
+    the semantics (core/tools/de_dense.cpp:1286-1293), tie-saturated and spread,
+    like core/tests/test_top2_merge.cpp."""
+    rng = np.random.default_rng(20260809)
+    for spread in (3, 12, 30000):
+        for _ in range(400):
+            G = int(rng.integers(1, 5))
+            D = min(256, G * 64 - int(rng.integers(0, 5)))
+            v = rng.integers(0, spread, 256) - spread // 2
+            v = np.clip(v, -16384, 16384)
+            kmax = int(rng.integers(-1, D))
+            keys = np.zeros((64, G), np.int64)
+            for g in range(G):
+                kk = g * 64 + np.arange(64)
+                ok = kk <= kmax
+                keys[ok, g] = ((v[kk[ok]] + 16384 + 1) << 8) | (255 - kk[ok])
+            p0 = np.zeros(64, np.int64)
+            p1 = np.zeros(64, np.int64)
+            for g in range(G):
+                k = keys[:, g]
+                p1 = np.maximum(p1, np.minimum(p0, k))
+                p0 = np.maximum(p0, k)
+            lanes = np.arange(64)
+            for off in (1, 2, 4, 8, 16, 32):
+                o0, o1 = p0[lanes ^ off], p1[lanes ^ off]
+                n1 = np.maximum(np.minimum(p0, o0), np.maximum(p1, o1))
+                p0 = np.maximum(p0, o0)
+                p1 = n1
+            assert (p0 == p0[0]).all() and (p1 == p1[0]).all()
+            b0, i0, b1, i1 = _seq_top2(v, kmax)
+            got = (((p0[0] >> 8) - 1 - 16384) if p0[0] else -32768,
+                   (255 - (p0[0] & 255)) if p0[0] else -1,
+                   ((p1[0] >> 8) - 1 - 16384) if p1[0] else -32768,
+                   (255 - (p1[0] & 255)) if p1[0] else -1)
+            assert got == (b0, i0, b1, i1)
