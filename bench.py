@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run one plain device-to-device copy of MIB MiB (the "
+                         "known-size kernel that scales rocprofv3 FETCH/WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -107,6 +110,13 @@ def main():
 
     L, R, _ = synth.slanted_pair(W, H, seed=2)
     dev = torch.device("cuda:0")
+    if a.calibrate_copy:
+        n = a.calibrate_copy << 20
+        src = torch.ones(n, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     dL = torch.from_numpy(L).to(dev)
     dR = torch.from_numpy(R).to(dev)
     disp = torch.empty((H, W), dtype=torch.float32, device=dev)

@@ -55,6 +55,7 @@ SIGNATURES = {
     "de_dense_volume": (I, [P, P, P, I, I, C.POINTER(DeCfgC), I, I, P]),
     "de_dense_candidates": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P]),
     "de_ctx_set_profiling": (I, [P, I]),
+    "de_ctx_set_variants": (I, [P, I, I]),
     "de_ctx_stage_ms": (I, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
 }
 

@@ -258,10 +258,12 @@ int Ctx::run_frames(const FrameDesc* fds, int n, hipStream_t s, int stage_mask) 
   if ((stage_mask & 1) && (e = launch_prep(bv, items(kStagePrep), hw, hh, s)) != hipSuccess)
     return hip_fail(e, "k_prep");
   if (prof) DE_HIP(hipEventRecord(ev.e[1], s));
-  if ((stage_mask & 2) && (e = launch_row(bv, items(kStageRow), lds_row, s)) != hipSuccess)
+  const int rowv = pick_row_variant(fds[0].W, fds[0].H, gmax);
+  const int colv = pick_col_variant(fds[0].W, fds[0].H, gmax);
+  if ((stage_mask & 2) && (e = launch_row(bv, items(kStageRow), lds_row, rowv, s)) != hipSuccess)
     return hip_fail(e, "k_row");
   if (prof) DE_HIP(hipEventRecord(ev.e[2], s));
-  if ((stage_mask & 4) && (e = launch_col(bv, items(kStageCol), gmax, s)) != hipSuccess)
+  if ((stage_mask & 4) && (e = launch_col(bv, items(kStageCol), gmax, colv, s)) != hipSuccess)
     return hip_fail(e, "k_col");
   if (prof) DE_HIP(hipEventRecord(ev.e[3], s));
   if ((stage_mask & 8) && (e = launch_solve(bv, items(kStageSolve), lds_solve, s)) != hipSuccess)
@@ -285,6 +287,17 @@ int Ctx::dense_dev(const uint8_t* dL, const uint8_t* dR, int W, int H, const de_
   if (int rc = frame_desc(c, W, H, dL, dR, disp, margin, cand6, nullptr, nullptr, sp, &fd))
     return rc;
   return run_frames(&fd, 1, s, stage_mask);
+}
+
+// Defaults until the autotuner has measured a shape: the column pass keeps 32
+// rows (G <= 2) or 16 rows in flight, the row pass 32 columns.
+int Ctx::pick_row_variant(int, int, int) const {
+  if (force_row_variant >= 0) return force_row_variant;
+  return 2;
+}
+int Ctx::pick_col_variant(int, int, int G) const {
+  if (force_col_variant >= 0) return force_col_variant;
+  return G <= 2 ? 2 : 1;
 }
 
 int Ctx::take_evset(EvSet* out) {
@@ -347,6 +360,18 @@ void de_default_cfg(de_cfg* c) {
 }
 
 const char* de_last_error(void) { return g_err.c_str(); }
+
+int de_ctx_set_variants(de_ctx* c, int row_variant, int col_variant) {
+  if (!c) return fail(DE_ERR_ARG, "null ctx");
+  if (row_variant < -1 || row_variant >= kNumRowVariants || col_variant < -1 ||
+      col_variant >= kNumColVariants)
+    return fail(DE_ERR_ARG, "variant out of range (row %d of %d, col %d of %d)", row_variant,
+                kNumRowVariants, col_variant, kNumColVariants);
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  c->c.force_row_variant = row_variant;
+  c->c.force_col_variant = col_variant;
+  return DE_OK;
+}
 
 int de_ctx_set_profiling(de_ctx* c, int on) {
   if (!c) return fail(DE_ERR_ARG, "null ctx");

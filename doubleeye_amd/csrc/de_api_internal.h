@@ -51,6 +51,11 @@ struct Ctx {
   Slot slots[kSlots];
   int next_slot = 0;
   bool store_vol = false;
+  // launch variants (prefetch depths) for the row and column passes; -1 = the
+  // tuned or default choice for the shape
+  int force_row_variant = -1, force_col_variant = -1;
+  int pick_row_variant(int W, int H, int G) const;
+  int pick_col_variant(int W, int H, int G) const;
   // optional per-stage timing: hipEvents recorded between the stage launches
   struct EvSet { hipEvent_t e[kNumStages + 1]; };
   bool profiling = false;
@@ -79,8 +84,8 @@ struct Ctx {
 size_t row_lds_bytes(int W, int nbits, int sc_shift);
 size_t solve_lds_bytes(int W);
 hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_t s);
-hipError_t launch_row(const BatchView& b, int items, size_t lds, hipStream_t s);
-hipError_t launch_col(const BatchView& b, int items, int gmax, hipStream_t s);
+hipError_t launch_row(const BatchView& b, int items, size_t lds, int variant, hipStream_t s);
+hipError_t launch_col(const BatchView& b, int items, int gmax, int variant, hipStream_t s);
 hipError_t launch_solve(const BatchView& b, int items, size_t lds, hipStream_t s);
 hipError_t launch_census_plane(const uint8_t* img, int W, int H, int hw, int hh,
                                uint64_t* out, hipStream_t s);

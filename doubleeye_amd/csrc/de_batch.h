@@ -10,6 +10,12 @@ namespace de {
 
 enum Stage { kStagePrep = 0, kStageRow = 1, kStageCol = 2, kStageSolve = 3, kNumStages = 4 };
 
+// Prefetch-depth launch variants of the row and column passes.
+constexpr int kNumRowVariants = 4;
+constexpr int kNumColVariants = 3;
+extern const int kRowVariants[kNumRowVariants];
+extern const int kColVariants[kNumColVariants];
+
 constexpr int kPrepTileW = 64;   // census/coefficient tile: 64 x 8 pixels
 constexpr int kPrepTileH = 8;
 

@@ -17,7 +17,24 @@
 
 #include <stdint.h>
 
+// Explicit address spaces: read-only per-launch data goes through the constant
+// space (uniform addresses become scalar loads, freely hoisted), everything a
+// kernel writes through the global space (global_* rather than flat_*
+// instructions, so LDS waits never wait on outstanding stores).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define DE_GLOBAL __attribute__((address_space(1)))
+#define DE_CONST __attribute__((address_space(4)))
+#else
+#define DE_GLOBAL
+#define DE_CONST
+#endif
+
 namespace de {
+
+template <class T>
+__host__ __device__ inline DE_GLOBAL T* gptr(T* p) { return (DE_GLOBAL T*)(p); }
+template <class T>
+__host__ __device__ inline const DE_CONST T* cptr(const T* p) { return (const DE_CONST T*)(p); }
 
 constexpr int kWave = 64;          // CDNA wavefront
 constexpr int kQ = 14;
@@ -33,7 +50,7 @@ struct FrameDesc {
   const uint8_t* L;      // left  image, W*H
   const uint8_t* R;      // right image, W*H
   uint32_t* lrec;        // left record per pixel, 4 words: census lo, census hi,
-                         //   2*L | (2*ax) << 16, 0  (read with scalar loads)
+                         //   2*L | (2*ax) << 16, 65536 - 2*ax (scalar loads)
   uint64_t* cr;          // right census plane, W*H
   uint32_t* ayt;         // 2*ay, transposed [x][y] (scalar loads down a column)
   uint16_t* vol;         // aggregated volume v', [y][x][Dpad]

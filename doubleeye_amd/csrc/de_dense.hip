@@ -15,6 +15,10 @@
 //
 // Volume layout: v' u16 at vol[(y*W + x)*Dpad + k], Dpad = 64*G. A wave owns 64
 // consecutive k, so every volume access is one 128-byte coalesced segment.
+//
+// The recurrences are sequential along their scan line, so a wave runs 64 (row
+// pass) or 64*G (column pass) independent chains; latency is hidden by
+// prefetching the scan U steps ahead, not by occupancy.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -35,6 +39,12 @@ __device__ __forceinline__ uint32_t rf_acc(uint32_t a2, uint32_t c) {
 }
 __device__ __forceinline__ uint16_t hi16(uint32_t t2) { return uint16_t(t2 >> 16); }
 
+__device__ __forceinline__ uint32_t med3(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 __device__ __forceinline__ int local_item(const BatchView& b, int stage, int item, int f) {
   return item - b.start[stage * (b.n + 1) + f];
 }
@@ -52,20 +62,25 @@ __global__ __launch_bounds__(256) void k_prep(BatchView b) {
   __shared__ uint16_t srf[256];
   const int item = blockIdx.x;
   const int f = find_frame(b, kStagePrep, item);
-  const FrameDesc& F = b.frames[f];
+  const FrameDesc F = b.frames[f];
   const int local = local_item(b, kStagePrep, item, f);
   const int W = F.W, H = F.H;
   const int ntx = (W + kPrepTileW - 1) / kPrepTileW;
   const int x0 = (local % ntx) * kPrepTileW, y0 = (local / ntx) * kPrepTileH;
-  srf[threadIdx.x] = F.rf[threadIdx.x];
+  const DE_CONST uint8_t* Lg = cptr(F.L);
+  const DE_CONST uint8_t* Rg = cptr(F.R);
+  srf[threadIdx.x] = cptr(F.rf)[threadIdx.x];
   for (int i = threadIdx.x; i < TW * TH; i += 256) {
     const int ty = i / TW, tx = i - ty * TW;
     const int gx = x0 - HW + tx, gy = y0 - HH + ty;
     const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
-    sl[ty][tx] = in ? F.L[size_t(gy) * W + gx] : 0;
-    sr[ty][tx] = in ? F.R[size_t(gy) * W + gx] : 0;
+    sl[ty][tx] = in ? Lg[size_t(gy) * W + gx] : 0;
+    sr[ty][tx] = in ? Rg[size_t(gy) * W + gx] : 0;
   }
   __syncthreads();
+  DE_GLOBAL uint4* lrec = gptr(reinterpret_cast<uint4*>(F.lrec));
+  DE_GLOBAL uint64_t* cr = gptr(F.cr);
+  DE_GLOBAL uint32_t* ayt = gptr(F.ayt);
   const int lx = threadIdx.x & 63;
   for (int r = threadIdx.x >> 6; r < kPrepTileH; r += 4) {
     const int x = x0 + lx, y = y0 + r;
@@ -93,10 +108,10 @@ __global__ __launch_bounds__(256) void k_prep(BatchView b) {
     rec.x = uint32_t(vl);
     rec.y = uint32_t(vl >> 32);
     rec.z = uint32_t(2 * lc) | ((2u * ax) << 16);
-    rec.w = 0;
-    reinterpret_cast<uint4*>(F.lrec)[i] = rec;
-    F.cr[i] = vr;
-    F.ayt[size_t(x) * H + y] = 2u * ay;
+    rec.w = 65536u - 2u * ax;
+    lrec[i] = rec;
+    cr[i] = vr;
+    ayt[size_t(x) * H + y] = 2u * ay;
   }
 }
 
@@ -114,18 +129,26 @@ __device__ __forceinline__ uint32_t score_at(const uint8_t* smem, const uint64_t
   return *reinterpret_cast<const uint16_t*>(smem + ((h << sh) + ad2));
 }
 
+__device__ __forceinline__ uint32_t rf_step_rec(const uint4& rec, uint32_t t2, uint32_t c) {
+  // rec.z >> 16 = 2a, rec.w = 65536 - 2a (both wave-uniform)
+  const uint32_t a2 = rec.z >> 16;
+  return rf_chain(a2, t2, __umul24(rec.w, c) + 32768u);
+}
+
+// U: H-bwd prefetch depth (columns of forward values in flight per lane)
+template <int U>
 __global__ __launch_bounds__(64) void k_row(BatchView b) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int item = blockIdx.x;
   const int f = find_frame(b, kStageRow, item);
-  const FrameDesc& F = b.frames[f];
+  const FrameDesc F = b.frames[f];
   const int local = local_item(b, kStageRow, item, f);
   const int W = F.W, H = F.H, Dp = F.Dpad, G = F.G;
   const int y = local / G, g = local - y * G;
   const int lane = threadIdx.x;
   const int k = g * kWave + lane;
   const int d = F.dmin + k;
-  uint16_t* vcol = F.vol + size_t(y) * W * Dp + k;      // element x at vcol[x*Dp]
+  DE_GLOBAL uint16_t* vcol = gptr(F.vol) + size_t(y) * W * Dp + k;   // x at vcol[x*Dp]
   const int xlo = F.hw + F.dmin + g * kWave;             // lane 0 valid from here
   const int xend = W - F.hw;                             // left pixel valid below
   if (y < F.hh || y >= H - F.hh || xend <= xlo) {
@@ -139,13 +162,16 @@ __global__ __launch_bounds__(64) void k_row(BatchView b) {
   uint16_t* r2s = reinterpret_cast<uint16_t*>(crs + W);
   {
     uint16_t* tbl = reinterpret_cast<uint16_t*>(smem);
-    for (int i = lane; i < ntbl; i += kWave) tbl[i] = F.sc[i];
-    const uint64_t* crg = F.cr + size_t(y) * W;
-    const uint8_t* Rg = F.R + size_t(y) * W;
+    const DE_CONST uint16_t* sc = cptr(F.sc);
+#pragma unroll 8
+    for (int i = lane; i < ntbl; i += kWave) tbl[i] = sc[i];
+    const DE_CONST uint64_t* crg = cptr(F.cr) + size_t(y) * W;
+    const DE_CONST uint8_t* Rg = cptr(F.R) + size_t(y) * W;
+#pragma unroll 8
     for (int x = lane; x < W; x += kWave) { crs[x] = crg[x]; r2s[x] = uint16_t(2 * Rg[x]); }
   }
   __syncthreads();
-  const uint4* __restrict__ lr = reinterpret_cast<const uint4*>(F.lrec) + size_t(y) * W;
+  const DE_CONST uint4* lr = cptr(reinterpret_cast<const uint4*>(F.lrec)) + size_t(y) * W;
   const int sh = F.sc_shift + 1;
   const uint32_t T2 = 2u * uint32_t(F.ad_T);
   const int xa = xlo;                                    // xlo < xend here
@@ -158,26 +184,49 @@ __global__ __launch_bounds__(64) void k_row(BatchView b) {
     const uint4 rec = lr[x];
     const bool ok = x >= F.hw + d;
     const uint32_t c = ok ? score_at(smem, crs, r2s, rec, ok ? x - d : 0, sh, T2) : uint32_t(kOff);
-    const uint32_t a2 = __builtin_amdgcn_readfirstlane(rec.z >> 16);
-    t2 = rf_chain(a2, t2, rf_acc(a2, c));
+    t2 = rf_step_rec(rec, t2, c);
     vcol[size_t(x) * Dp] = hi16(t2);
   }
-#pragma unroll 4
-  for (int x = xb; x < xend; ++x) {                      // every lane inside its window
-    const uint4 rec = lr[x];
-    const uint32_t c = score_at(smem, crs, r2s, rec, x - d, sh, T2);
-    const uint32_t a2 = __builtin_amdgcn_readfirstlane(rec.z >> 16);
-    t2 = rf_chain(a2, t2, rf_acc(a2, c));
-    vcol[size_t(x) * Dp] = hi16(t2);
+  {
+    // scores of the next 8 columns are computed while the chain runs the
+    // current 8 (they only depend on LDS and the scalar records)
+    int x = xb;
+    if (x + 8 <= xend) {
+      uint32_t c[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) c[u] = score_at(smem, crs, r2s, lr[x + u], x + u - d, sh, T2);
+      for (; x + 16 <= xend; x += 8) {
+        uint32_t cn[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          cn[u] = score_at(smem, crs, r2s, lr[x + 8 + u], x + 8 + u - d, sh, T2);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          t2 = rf_step_rec(lr[x + u], t2, c[u]);
+          vcol[size_t(x + u) * Dp] = hi16(t2);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = cn[u];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        t2 = rf_step_rec(lr[x + u], t2, c[u]);
+        vcol[size_t(x + u) * Dp] = hi16(t2);
+      }
+      x += 8;
+    }
+    for (; x < xend; ++x) {
+      const uint4 rec = lr[x];
+      t2 = rf_step_rec(rec, t2, score_at(smem, crs, r2s, rec, x - d, sh, T2));
+      vcol[size_t(x) * Dp] = hi16(t2);
+    }
   }
   for (int x = xend; x < W; ++x) {                       // right border: zero scores
-    const uint32_t a2 = __builtin_amdgcn_readfirstlane(lr[x].z >> 16);
-    t2 = rf_chain(a2, t2, rf_acc(a2, uint32_t(kOff)));
+    t2 = rf_step_rec(lr[x], t2, uint32_t(kOff));
     vcol[size_t(x) * Dp] = hi16(t2);
   }
 
   // ---- H-bwd: g'[W-1] = f'[W-1]; coefficient of x+1 ----
-  constexpr int U = 8;
   int x = W - 2;
   uint32_t cur[U];
   // software pipeline: the U forward values of the next chunk are in flight
@@ -196,8 +245,7 @@ __global__ __launch_bounds__(64) void k_row(BatchView b) {
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t a2 = __builtin_amdgcn_readfirstlane(lr[x - u + 1].z >> 16);
-      t2 = rf_chain(a2, t2, rf_acc(a2, cur[u]));
+      t2 = rf_step_rec(lr[x - u + 1], t2, cur[u]);
       vcol[size_t(x - u) * Dp] = hi16(t2);
     }
     if (more) {
@@ -207,118 +255,177 @@ __global__ __launch_bounds__(64) void k_row(BatchView b) {
   }
   for (; x >= xa; --x) {
     const uint32_t c = vcol[size_t(x) * Dp];
-    const uint32_t a2 = __builtin_amdgcn_readfirstlane(lr[x + 1].z >> 16);
-    t2 = rf_chain(a2, t2, rf_acc(a2, c));
+    t2 = rf_step_rec(lr[x + 1], t2, c);
     vcol[size_t(x) * Dp] = hi16(t2);
   }
   for (; x >= 0; --x) {                                  // forward value is kOff here
-    const uint32_t a2 = __builtin_amdgcn_readfirstlane(lr[x + 1].z >> 16);
-    t2 = rf_chain(a2, t2, rf_acc(a2, uint32_t(kOff)));
+    t2 = rf_step_rec(lr[x + 1], t2, uint32_t(kOff));
     vcol[size_t(x) * Dp] = hi16(t2);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Stage 3: V-fwd + V-bwd + exact top-2 with sub-pixel neighbours, one wave per
-// column x holding all G disparity groups (G chains per lane). The V-bwd values
-// never go back to memory; the per-pixel output is one 16-byte record.
-__device__ __forceinline__ void top2_insert(uint32_t& p0, uint32_t& p1, uint32_t key) {
-  // distinct keys: new best = max, new runner-up = median of three
-  p1 = max(p1, min(p0, key));
-  p0 = max(p0, key);
-}
+// column x holding all G disparity groups (G chains per lane).
+//
+// V-bwd runs in blocks of 64 rows. While a block is filtered, every lane drops
+// its final values into an LDS block (row-major, groups packed two per word,
+// row stride padded by one word so the transposed reads below hit 64 distinct
+// banks). Then the block is read transposed: lane r owns row r of the block
+// and runs the top-2 over that row's disparities itself, on packed keys
+// ((v'+1) << 8 | 255-k), so ties resolve to the smaller k exactly as the
+// reference's ascending-k strictly-greater insertion. The neighbours k*-1 and
+// k*+1 are two more LDS reads. No cross-lane reduction is needed at all. The
+// V-bwd values never go back to memory; the output is one 16-byte record per
+// pixel.
 
-__device__ __forceinline__ void top2_merge(uint32_t& p0, uint32_t& p1, uint32_t o0, uint32_t o1) {
-  const uint32_t n1 = max(min(p0, o0), max(p1, o1));
-  p0 = max(p0, o0);
-  p1 = n1;
-}
-
-template <int GM>
+// U: rows prefetched ahead in both vertical passes (bytes in flight per wave =
+// U * 128 * G; a divisor of 64)
+template <int GM, int U>
 __global__ __launch_bounds__(64) void k_col(BatchView b) {
+  constexpr int NWL = (GM + 1) / 2;          // LDS words per lane per row
+  constexpr int RS = kWave * NWL + 1;        // padded row stride (words)
+  static_assert(kWave % U == 0, "U must divide the 64-row block");
+  __shared__ uint32_t rb[kWave * RS];
   const int item = blockIdx.x;
   const int f = find_frame(b, kStageCol, item);
-  const FrameDesc& F = b.frames[f];
+  const FrameDesc F = b.frames[f];
   const int x = local_item(b, kStageCol, item, f);
   const int W = F.W, H = F.H, Dp = F.Dpad, G = F.G;
   const int lane = threadIdx.x;
-  uint16_t* vx = F.vol + size_t(x) * Dp + lane;          // (y, g) at vx[y*ys + g*64]
-  const size_t ys = size_t(W) * Dp;
-  const uint32_t* __restrict__ ay = F.ayt + size_t(x) * H;
+  const ptrdiff_t ys = ptrdiff_t(W) * Dp;                // row stride (elements)
+  DE_GLOBAL uint16_t* const v0 = gptr(F.vol) + size_t(x) * Dp + lane;   // row 0, group 0
+  const DE_CONST uint32_t* ay = cptr(F.ayt) + size_t(x) * H;
+  DE_GLOBAL uint4* rec = gptr(reinterpret_cast<uint4*>(F.cand)) + x;   // row y at rec[y*W]
   uint32_t t[GM];
 #pragma unroll
-  for (int g = 0; g < GM; ++g) t[g] = g < G ? uint32_t(vx[g * kWave]) << 16 : 0u;
+  for (int g = 0; g < GM; ++g) t[g] = g < G ? uint32_t(v0[g * kWave]) << 16 : 0u;
 
-  // ---- V-fwd ----
-  for (int y = 1; y < H; ++y) {
-    const uint32_t a2 = __builtin_amdgcn_readfirstlane(ay[y]);
-    uint16_t* p = vx + size_t(y) * ys;
+  // rows [y0, y0 + dir*U) of the G groups at this lane's disparity
+  auto load_rows = [&](uint32_t (&dst)[U][GM], int y0, int dir) {
+    DE_GLOBAL uint16_t* p = v0 + ptrdiff_t(y0) * ys;
 #pragma unroll
-    for (int g = 0; g < GM; ++g) {
-      if (g < G) {
-        t[g] = rf_chain(a2, t[g], rf_acc(a2, p[g * kWave]));
-        p[g * kWave] = hi16(t[g]);
+    for (int u = 0; u < U; ++u) {
+      const int yy = y0 + dir * u;
+#pragma unroll
+      for (int g = 0; g < GM; ++g)
+        dst[u][g] = (yy >= 0 && yy < H && g < G) ? uint32_t(p[g * kWave]) : 0u;
+      p += dir * ys;
+    }
+  };
+
+  // ---- V-fwd over rows 1..H-1, U rows in flight ----
+  {
+    uint32_t cur[U][GM];
+    load_rows(cur, 1, 1);
+    DE_GLOBAL uint16_t* p = v0 + ys;
+    for (int y0 = 1; y0 < H; y0 += U) {
+      uint32_t nxt[U][GM];
+      const bool more = y0 + U < H;
+      if (more) load_rows(nxt, y0 + U, 1);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (y0 + u < H) {
+          const uint32_t a2 = ay[y0 + u];
+#pragma unroll
+          for (int g = 0; g < GM; ++g) {
+            if (g < G) {
+              t[g] = rf_chain(a2, t[g], rf_acc(a2, cur[u][g]));
+              p[g * kWave] = hi16(t[g]);
+            }
+          }
+        }
+        p += ys;
+      }
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int g = 0; g < GM; ++g) cur[u][g] = nxt[u][g];
       }
     }
   }
 
-  // ---- V-bwd + top-2 ----
+  // ---- V-bwd in blocks of 64 rows, transposed top-2 ----
   const int kmax = (x >= F.hw + F.dmin && x < W - F.hw) ? min(F.D - 1, x - F.hw - F.dmin) : -1;
-  uint4* rec = reinterpret_cast<uint4*>(F.cand) + x;    // row y at rec[y*W]
+  const int ylo = F.hh, yhi = H - F.hh;                  // rows with candidates
   if (lane == 0) rec[size_t(H - 1) * W] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
-  for (int y = H - 2; y >= 0; --y) {
-    const uint32_t a2 = __builtin_amdgcn_readfirstlane(ay[y + 1]);
-    const uint16_t* p = vx + size_t(y) * ys;
-    uint32_t v[GM];
+  uint32_t cur[U][GM];
+  load_rows(cur, H - 2, -1);
+  DE_GLOBAL uint16_t* p = v0 + ptrdiff_t(H - 2) * ys;
+  for (int yb = H - 2; yb >= 0; yb -= kWave) {          // block: rows yb, yb-1, ..., yb-63
+    const int nrow = min(kWave, yb + 1);
+    for (int u0 = 0; u0 < nrow; u0 += U) {
+      const int y0 = yb - u0;
+      uint32_t nxt[U][GM];
+      const bool more = y0 - U >= 0;
+      if (more) load_rows(nxt, y0 - U, -1);
 #pragma unroll
-    for (int g = 0; g < GM; ++g) {
-      if (g < G) {
-        t[g] = rf_chain(a2, t[g], rf_acc(a2, p[g * kWave]));
-        v[g] = t[g] >> 16;
-        if (F.store_vol) const_cast<uint16_t*>(p)[g * kWave] = uint16_t(v[g]);
-      } else {
-        v[g] = 0;
+      for (int u = 0; u < U; ++u) {
+        const int yy = y0 - u;
+        if (yy >= 0 && u0 + u < nrow) {
+          const uint32_t a2 = ay[yy + 1];
+          uint32_t v[GM];
+#pragma unroll
+          for (int g = 0; g < GM; ++g) {
+            if (g < G) {
+              t[g] = rf_chain(a2, t[g], rf_acc(a2, cur[u][g]));
+              v[g] = t[g] >> 16;
+              if (F.store_vol) p[g * kWave] = uint16_t(v[g]);
+            } else {
+              v[g] = 0u;
+            }
+          }
+#pragma unroll
+          for (int w = 0; w < NWL; ++w) {
+            const uint32_t lo = v[2 * w];
+            const uint32_t hi = (2 * w + 1 < GM) ? v[2 * w + 1] : 0u;
+            rb[(u0 + u) * RS + w * kWave + lane] = lo | (hi << 16);
+          }
+        }
+        p -= ys;
+      }
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int g = 0; g < GM; ++g) cur[u][g] = nxt[u][g];
       }
     }
-    if (y < F.hh || y >= H - F.hh || kmax < 0) {
-      if (lane == 0) rec[size_t(y) * W] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
-      continue;
-    }
-    uint32_t p0 = 0, p1 = 0;
-#pragma unroll
-    for (int g = 0; g < GM; ++g) {
-      const int kk = g * kWave + lane;
-      if (g < G && kk <= kmax) top2_insert(p0, p1, pack_key(v[g], kk));
-    }
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const uint32_t o0 = __shfl_xor(p0, off);
-      const uint32_t o1 = __shfl_xor(p1, off);
-      top2_merge(p0, p1, o0, o1);
-    }
-    p0 = __builtin_amdgcn_readfirstlane(p0);
-    p1 = __builtin_amdgcn_readfirstlane(p1);
-    uint32_t nb = 0xFFFFFFFFu;
-    if (p0) {
-      const int ks = key_k(p0);
-      uint32_t nm = 0xFFFFu, np = 0xFFFFu;
-      if (ks >= 1) {
-        const int km = ks - 1, gm = km >> 6;
-        uint32_t s = 0;
-#pragma unroll
-        for (int g = 0; g < GM; ++g) s = (g == gm) ? v[g] : s;
-        nm = __builtin_amdgcn_readlane(s, km & 63);
+    __syncthreads();
+    // lane r: top-2 of row yb - r over k in [0, kmax]
+    if (lane < nrow) {
+      const int yy = yb - lane;
+      uint32_t p0 = 0u, p1 = 0u, nbw = 0xFFFFFFFFu;
+      if (yy >= ylo && yy < yhi && kmax >= 0) {
+        const uint32_t* row = rb + lane * RS;
+        const int nw = min(NWL * kWave, (kmax >> 7) * kWave + kWave);
+        for (int j = 0; j < nw; ++j) {
+          const uint32_t wv = row[j];
+          const int klo = ((j >> 6) << 7) + (j & 63);   // word j holds k = klo and klo + 64
+          const uint32_t key_lo = (wv << 8 & 0xFFFF00u) + uint32_t(256 + 255 - klo);
+          const uint32_t key_hi = (wv >> 8 & 0xFFFF00u) + uint32_t(256 + 255 - 64 - klo);
+          const uint32_t a = klo <= kmax ? key_lo : 0u;
+          const uint32_t c = klo + 64 <= kmax ? key_hi : 0u;
+          p1 = med3(p0, p1, a);
+          p0 = max(p0, a);
+          p1 = med3(p0, p1, c);
+          p0 = max(p0, c);
+        }
+        if (p0) {
+          const int ks = key_k(p0);
+          auto val = [&](int k) -> uint32_t {
+            const uint32_t wv = row[((k >> 7) << 6) + (k & 63)];
+            return ((k >> 6) & 1) ? (wv >> 16) : (wv & 0xFFFFu);
+          };
+          const uint32_t nm = ks >= 1 ? val(ks - 1) : 0xFFFFu;
+          const uint32_t np = ks + 1 <= kmax ? val(ks + 1) : 0xFFFFu;
+          nbw = nm | (np << 16);
+        }
       }
-      if (ks + 1 <= kmax) {
-        const int kp = ks + 1, gp = kp >> 6;
-        uint32_t s = 0;
-#pragma unroll
-        for (int g = 0; g < GM; ++g) s = (g == gp) ? v[g] : s;
-        np = __builtin_amdgcn_readlane(s, kp & 63);
-      }
-      nb = nm | (np << 16);
+      rec[size_t(yy) * W] = make_uint4(p0, p1, nbw, 0u);
     }
-    if (lane == 0) rec[size_t(y) * W] = make_uint4(p0, p1, nb, 0u);
+    __syncthreads();
   }
 }
 
@@ -374,17 +481,37 @@ hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_row(const BatchView& b, int items, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL(k_row, dim3(items), dim3(64), lds, s, b);
+// Prefetch-depth variants (chosen per shape by the autotuner, de_tune.cpp).
+const int kRowVariants[kNumRowVariants] = {8, 16, 32, 48};
+const int kColVariants[kNumColVariants] = {8, 16, 32};
+
+hipError_t launch_row(const BatchView& b, int items, size_t lds, int variant, hipStream_t s) {
+  switch (variant) {
+    case 0: hipLaunchKernelGGL(k_row<8>, dim3(items), dim3(64), lds, s, b); break;
+    case 1: hipLaunchKernelGGL(k_row<16>, dim3(items), dim3(64), lds, s, b); break;
+    case 2: hipLaunchKernelGGL(k_row<32>, dim3(items), dim3(64), lds, s, b); break;
+    case 3: hipLaunchKernelGGL(k_row<48>, dim3(items), dim3(64), lds, s, b); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_col(const BatchView& b, int items, int gmax, hipStream_t s) {
+template <int GM>
+static void launch_col_g(const BatchView& b, int items, int variant, hipStream_t s) {
+  switch (variant) {
+    case 0: hipLaunchKernelGGL((k_col<GM, 8>), dim3(items), dim3(64), 0, s, b); break;
+    case 1: hipLaunchKernelGGL((k_col<GM, 16>), dim3(items), dim3(64), 0, s, b); break;
+    default: hipLaunchKernelGGL((k_col<GM, 32>), dim3(items), dim3(64), 0, s, b); break;
+  }
+}
+
+hipError_t launch_col(const BatchView& b, int items, int gmax, int variant, hipStream_t s) {
+  if (variant < 0 || variant >= kNumColVariants) return hipErrorInvalidValue;
   switch (gmax) {
-    case 1: hipLaunchKernelGGL(k_col<1>, dim3(items), dim3(64), 0, s, b); break;
-    case 2: hipLaunchKernelGGL(k_col<2>, dim3(items), dim3(64), 0, s, b); break;
-    case 3: hipLaunchKernelGGL(k_col<3>, dim3(items), dim3(64), 0, s, b); break;
-    case 4: hipLaunchKernelGGL(k_col<4>, dim3(items), dim3(64), 0, s, b); break;
+    case 1: launch_col_g<1>(b, items, variant, s); break;
+    case 2: launch_col_g<2>(b, items, variant, s); break;
+    case 3: launch_col_g<3>(b, items, variant, s); break;
+    case 4: launch_col_g<4>(b, items, variant, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

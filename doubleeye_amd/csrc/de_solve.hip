@@ -112,14 +112,16 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
   extern __shared__ __align__(16) uint8_t smem_raw[];
   const int item = blockIdx.x;
   const int f = find_frame(b, kStageSolve, item);
-  const FrameDesc& F = b.frames[f];
+  const FrameDesc F = b.frames[f];
   const int y = item - b.start[kStageSolve * (b.n + 1) + f];
   const int W = F.W, dmin = F.dmin;
   const int tid = threadIdx.x, nt = blockDim.x;
   const float lambda = F.lambda, gamma = F.gamma, damping = F.damping;
   const float keep = 1.f - damping;
   SolveSmem S = carve(smem_raw, W);
-  const uint4* recs = reinterpret_cast<const uint4*>(F.cand) + size_t(y) * W;
+  const DE_CONST uint4* recs = cptr(reinterpret_cast<const uint4*>(F.cand)) + size_t(y) * W;
+  DE_GLOBAL float* g_disp = gptr(F.disp);
+  DE_GLOBAL float* g_margin = gptr(F.margin);
   const size_t row = size_t(y) * W;
 
   // --- candidates, margin, bucket counts ---
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
     }
     S.cn[x] = uint8_t(n);
     const float alt = (n < 2) ? lambda : fmax_ref(S.cs[2 * x + 1], lambda);
-    F.margin[row + x] = (n == 0) ? 0.f : S.cs[2 * x] - alt;
+    g_margin[row + x] = (n == 0) ? 0.f : S.cs[2 * x] - alt;
     S.beta[2 * x] = S.beta[2 * x + 1] = 0.f;
     S.rho[2 * x] = S.rho[2 * x + 1] = 0.f;
     for (int j = 0; j < n; ++j) {
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
       const float bel = (F.iters > 0) ? (S.beta[2 * x + j] + S.rho[2 * x + j] - c) : c;
       if (bel > bst) { bst = bel; bj = j; }
     }
-    if (bj >= 0 && F.min_margin > 0.f && F.margin[row + x] < F.min_margin) bj = -1;
+    if (bj >= 0 && F.min_margin > 0.f && g_margin[row + x] < F.min_margin) bj = -1;
     if (bj >= 0 && F.min_ratio > 0.f) {
       const float s2 = (S.cn[x] > 1) ? fmax_ref(S.cs[2 * x + 1], lambda) : lambda;
       const float c1 = fmax_ref(1.f - S.cs[2 * x], 1e-6f);
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
         out = float(d) + off;
       }
     }
-    F.disp[row + x] = out;
+    g_disp[row + x] = out;
     const bool fin = out == out;
     float lrc2 = __int_as_float(0x7fc00000);
     if (fin) {
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
     }
     const int n = S.cn[x];
     if (F.cand6) {
-      float* c6 = F.cand6 + row + x;
+      DE_GLOBAL float* c6 = gptr(F.cand6) + row + x;
       c6[0] = n > 0 ? S.cs[2 * x] : kNone;
       c6[WH] = n > 1 ? S.cs[2 * x + 1] : kNone;
       c6[2 * WH] = n > 0 ? float(dmin + S.cd[2 * x]) : kNone;
@@ -300,11 +302,11 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
       c6[5 * WH] = lrc2;
     }
     if (F.q4) {
-      F.q4[row + x] = (fin && out > 0.f) ? int16_t(roundf(out * 16.0f)) : int16_t(-32768);
+      gptr(F.q4)[row + x] = (fin && out > 0.f) ? int16_t(roundf(out * 16.0f)) : int16_t(-32768);
     }
     if (F.conf) {
       const bool has = fin && out > 0.f;
-      F.conf[row + x] = has ? uint8_t(roundf(confidence_dev(n > 0 ? S.cs[2 * x] : kNone,
+      gptr(F.conf)[row + x] = has ? uint8_t(roundf(confidence_dev(n > 0 ? S.cs[2 * x] : kNone,
                                                             n > 1 ? S.cs[2 * x + 1] : kNone,
                                                             lambda, (lrc2 == lrc2) ? lrc2 : 0.f,
                                                             1.f) * 255.f))

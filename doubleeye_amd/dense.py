@@ -142,6 +142,9 @@ class Context:
         check(lib().de_census_dev(self.h, C.c_void_p(d_img), W, H, hw, hh, C.c_void_p(d_out),
                                   C.c_void_p(stream) if stream else None))
 
+    def set_variants(self, row: int = -1, col: int = -1):
+        check(lib().de_ctx_set_variants(self.h, row, col))
+
     def set_profiling(self, on: bool):
         check(lib().de_ctx_set_profiling(self.h, int(on)))
 
