@@ -193,7 +193,8 @@ int Ctx::frame_desc(const de_cfg& c, int W, int H, const uint8_t* L, const uint8
 size_t scratch_bytes(int W, int H, int D) {
   const size_t WH = size_t(W) * H;
   const int Dpad = ((D + kWave - 1) / kWave) * kWave;
-  return WH * (16 + 8 + 4 + 16) + WH * size_t(Dpad) * 2 + 5 * 256;
+  const size_t vol_rows = size_t(H) + 2 * kVolPadRows;
+  return WH * (16 + 8 + 4 + 16) + vol_rows * W * size_t(Dpad) * 2 + 5 * 256;
 }
 
 // Carve one frame's scratch out of a device block.
@@ -207,13 +208,22 @@ Scratch carve_scratch(void* base, int W, int H, int D) {
   s.cr = reinterpret_cast<uint64_t*>(take(WH * 8));
   s.ayt = reinterpret_cast<uint32_t*>(take(WH * 4));
   s.cand = reinterpret_cast<uint32_t*>(take(WH * 16));
-  s.vol = reinterpret_cast<uint16_t*>(take(WH * size_t(Dpad) * 2));
+  // the volume keeps kVolPadRows rows of padding above and below the frame
+  const size_t row_elems = size_t(W) * Dpad;
+  uint16_t* v = reinterpret_cast<uint16_t*>(take((size_t(H) + 2 * kVolPadRows) * row_elems * 2));
+  s.vol = v + kVolPadRows * row_elems;
   return s;
 }
 
 // Launch every stage over `n` frames (stage_mask selects stages, for tests).
-int Ctx::run_frames(const FrameDesc* fds, int n, hipStream_t s, int stage_mask) {
+int Ctx::run_frames(const FrameDesc* fds_in, int n, hipStream_t s, int stage_mask) {
   if (n < 1) return fail(DE_ERR_ARG, "empty batch");
+  // frames ordered by disparity-group count, so each column-pass bucket (one
+  // kernel instantiation per G) is a contiguous range of work items
+  std::vector<FrameDesc> sorted(fds_in, fds_in + n);
+  std::stable_sort(sorted.begin(), sorted.end(),
+                   [](const FrameDesc& a, const FrameDesc& b) { return a.G < b.G; });
+  const FrameDesc* fds = sorted.data();
   std::vector<int> start(size_t(kNumStages) * (n + 1));
   int gmax = 1;
   size_t lds_row = 0, lds_solve = 0;
@@ -263,8 +273,17 @@ int Ctx::run_frames(const FrameDesc* fds, int n, hipStream_t s, int stage_mask) 
   if ((stage_mask & 2) && (e = launch_row(bv, items(kStageRow), lds_row, rowv, s)) != hipSuccess)
     return hip_fail(e, "k_row");
   if (prof) DE_HIP(hipEventRecord(ev.e[2], s));
-  if ((stage_mask & 4) && (e = launch_col(bv, items(kStageCol), gmax, colv, s)) != hipSuccess)
-    return hip_fail(e, "k_col");
+  if (stage_mask & 4) {
+    for (int i = 0; i < n;) {
+      int j = i;
+      while (j < n && fds[j].G == fds[i].G) ++j;
+      const int base = start[size_t(kStageCol) * (n + 1) + i];
+      const int cnt = start[size_t(kStageCol) * (n + 1) + j] - base;
+      if ((e = launch_col(bv, base, cnt, fds[i].G, fds[i].store_vol != 0, colv, s)) != hipSuccess)
+        return hip_fail(e, "k_col");
+      i = j;
+    }
+  }
   if (prof) DE_HIP(hipEventRecord(ev.e[3], s));
   if ((stage_mask & 8) && (e = launch_solve(bv, items(kStageSolve), lds_solve, s)) != hipSuccess)
     return hip_fail(e, "k_solve");
@@ -289,15 +308,15 @@ int Ctx::dense_dev(const uint8_t* dL, const uint8_t* dR, int W, int H, const de_
   return run_frames(&fd, 1, s, stage_mask);
 }
 
-// Defaults until the autotuner has measured a shape: the column pass keeps 32
-// rows (G <= 2) or 16 rows in flight, the row pass 32 columns.
+// Defaults until the autotuner has measured a shape: 16 rows in flight in the
+// column pass, 32 columns in the row pass's backward scan.
 int Ctx::pick_row_variant(int, int, int) const {
   if (force_row_variant >= 0) return force_row_variant;
   return 2;
 }
 int Ctx::pick_col_variant(int, int, int G) const {
   if (force_col_variant >= 0) return force_col_variant;
-  return G <= 2 ? 2 : 1;
+  return 1;
 }
 
 int Ctx::take_evset(EvSet* out) {

@@ -85,7 +85,8 @@ size_t row_lds_bytes(int W, int nbits, int sc_shift);
 size_t solve_lds_bytes(int W);
 hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_t s);
 hipError_t launch_row(const BatchView& b, int items, size_t lds, int variant, hipStream_t s);
-hipError_t launch_col(const BatchView& b, int items, int gmax, int variant, hipStream_t s);
+hipError_t launch_col(const BatchView& b, int base, int items, int G, bool store, int variant,
+                      hipStream_t s);
 hipError_t launch_solve(const BatchView& b, int items, size_t lds, hipStream_t s);
 hipError_t launch_census_plane(const uint8_t* img, int W, int H, int hw, int hh,
                                uint64_t* out, hipStream_t s);

@@ -16,6 +16,9 @@ constexpr int kNumColVariants = 3;
 extern const int kRowVariants[kNumRowVariants];
 extern const int kColVariants[kNumColVariants];
 
+// Rows of padding above and below a frame's volume (prefetch runs ahead into it).
+constexpr int kVolPadRows = 64;
+
 constexpr int kPrepTileW = 64;   // census/coefficient tile: 64 x 8 pixels
 constexpr int kPrepTileH = 8;
 

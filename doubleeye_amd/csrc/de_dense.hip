@@ -34,10 +34,21 @@ __device__ __forceinline__ uint32_t rf_chain(uint32_t a2, uint32_t t2prev, uint3
   asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[0,1,0,0]" : "=v"(r) : "s"(a2), "v"(t2prev), "v"(acc));
   return r;
 }
+__device__ __forceinline__ uint32_t rf_chain_v(uint32_t a2, uint32_t t2prev, uint32_t acc) {
+  uint32_t r;
+  asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[0,1,0,0]" : "=v"(r) : "v"(a2), "v"(t2prev), "v"(acc));
+  return r;
+}
 __device__ __forceinline__ uint32_t rf_acc(uint32_t a2, uint32_t c) {
   return __umul24(65536u - a2, c) + 32768u;
 }
 __device__ __forceinline__ uint16_t hi16(uint32_t t2) { return uint16_t(t2 >> 16); }
+
+__device__ __forceinline__ uint32_t sad_u32(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_sad_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
 
 __device__ __forceinline__ uint32_t med3(uint32_t a, uint32_t b, uint32_t c) {
   uint32_t r;
@@ -117,22 +128,22 @@ __global__ __launch_bounds__(256) void k_prep(BatchView b) {
 
 // ---------------------------------------------------------------------------
 // Stage 2: graded score + H-fwd + H-bwd for one row and 64 disparities.
-// Lane = disparity; the left record of pixel x is wave-uniform (scalar loads),
-// the right census/intensity at x - d comes from the LDS-staged row.
-// Dynamic LDS: score table | right census row (u64) | 2*R row (u16).
+// Lane = disparity. The row's left records (wave-uniform per x, broadcast
+// reads) and its right census/intensity (read at x - d per lane) are staged
+// in LDS once, with coalesced loads.
 __device__ __forceinline__ uint32_t score_at(const uint8_t* smem, const uint64_t* crs,
                                              const uint16_t* r2s, uint4 lrec, int j,
                                              int sh, uint32_t T2) {
   const uint64_t cr = crs[j];
   const uint32_t h = __popc(lrec.x ^ uint32_t(cr)) + __popc(lrec.y ^ uint32_t(cr >> 32));
-  const uint32_t ad2 = min(__usad(lrec.z & 0xFFFFu, uint32_t(r2s[j]), 0u), T2);
+  const uint32_t ad2 = min(sad_u32(lrec.z & 0xFFFFu, uint32_t(r2s[j]), 0u), T2);
   return *reinterpret_cast<const uint16_t*>(smem + ((h << sh) + ad2));
 }
 
 __device__ __forceinline__ uint32_t rf_step_rec(const uint4& rec, uint32_t t2, uint32_t c) {
-  // rec.z >> 16 = 2a, rec.w = 65536 - 2a (both wave-uniform)
+  // rec.z >> 16 = 2a, rec.w = 65536 - 2a (wave-uniform, read from LDS)
   const uint32_t a2 = rec.z >> 16;
-  return rf_chain(a2, t2, __umul24(rec.w, c) + 32768u);
+  return rf_chain_v(a2, t2, __umul24(rec.w, c) + 32768u);
 }
 
 // U: H-bwd prefetch depth (columns of forward values in flight per lane)
@@ -145,47 +156,56 @@ __global__ __launch_bounds__(64) void k_row(BatchView b) {
   const int local = local_item(b, kStageRow, item, f);
   const int W = F.W, H = F.H, Dp = F.Dpad, G = F.G;
   const int y = local / G, g = local - y * G;
-  const int lane = threadIdx.x;
-  const int k = g * kWave + lane;
-  const int d = F.dmin + k;
-  DE_GLOBAL uint16_t* vcol = gptr(F.vol) + size_t(y) * W * Dp + k;   // x at vcol[x*Dp]
+  const uint32_t lane = threadIdx.x & 63;
+  const int d = F.dmin + g * kWave + int(lane);
+  // Wave-uniform pointer to column x of this row and group; lane l owns
+  // p[l]. Every access is scalar base + lane offset, stepped by Dp.
+  DE_GLOBAL uint16_t* const x0p = gptr(F.vol) + size_t(y) * W * Dp + g * kWave;
   const int xlo = F.hw + F.dmin + g * kWave;             // lane 0 valid from here
   const int xend = W - F.hw;                             // left pixel valid below
   if (y < F.hh || y >= H - F.hh || xend <= xlo) {
     // every score of this row is zero -> the filtered row is zero (v' = kOff)
-    for (int x = 0; x < W; ++x) vcol[size_t(x) * Dp] = uint16_t(kOff);
+    DE_GLOBAL uint16_t* p = x0p;
+    for (int x = 0; x < W; ++x, p += Dp) p[lane] = uint16_t(kOff);
     return;
   }
   const int nb = (2 * F.hw + 1) * (2 * F.hh + 1) - 1;
   const int ntbl = (nb + 1) << F.sc_shift;
-  uint64_t* crs = reinterpret_cast<uint64_t*>(smem + ((ntbl * 2 + 15) & ~15));
+  // LDS: score table | left records (uint4) | right census (u64) | 2*R (u16)
+  uint4* lr = reinterpret_cast<uint4*>(smem + ((ntbl * 2 + 15) & ~15));
+  uint64_t* crs = reinterpret_cast<uint64_t*>(lr + W);
   uint16_t* r2s = reinterpret_cast<uint16_t*>(crs + W);
   {
     uint16_t* tbl = reinterpret_cast<uint16_t*>(smem);
     const DE_CONST uint16_t* sc = cptr(F.sc);
 #pragma unroll 8
     for (int i = lane; i < ntbl; i += kWave) tbl[i] = sc[i];
+    const DE_CONST uint4* lrg = cptr(reinterpret_cast<const uint4*>(F.lrec)) + size_t(y) * W;
     const DE_CONST uint64_t* crg = cptr(F.cr) + size_t(y) * W;
     const DE_CONST uint8_t* Rg = cptr(F.R) + size_t(y) * W;
 #pragma unroll 8
-    for (int x = lane; x < W; x += kWave) { crs[x] = crg[x]; r2s[x] = uint16_t(2 * Rg[x]); }
+    for (int x = lane; x < W; x += kWave) {
+      lr[x] = lrg[x];
+      crs[x] = crg[x];
+      r2s[x] = uint16_t(2 * Rg[x]);
+    }
   }
   __syncthreads();
-  const DE_CONST uint4* lr = cptr(reinterpret_cast<const uint4*>(F.lrec)) + size_t(y) * W;
   const int sh = F.sc_shift + 1;
   const uint32_t T2 = 2u * uint32_t(F.ad_T);
   const int xa = xlo;                                    // xlo < xend here
   const int xb = max(xa, min(xlo + kWave - 1, xend));
 
   // ---- H-fwd: x in [0, xa) every score is zero, so f' stays kOff ----
-  for (int x = 0; x < xa; ++x) vcol[size_t(x) * Dp] = uint16_t(kOff);
+  DE_GLOBAL uint16_t* pw = x0p;
+  for (int x = 0; x < xa; ++x, pw += Dp) pw[lane] = uint16_t(kOff);
   uint32_t t2 = uint32_t(kOff) << 16;
-  for (int x = xa; x < xb; ++x) {                        // ragged band: per-lane window
+  for (int x = xa; x < xb; ++x, pw += Dp) {              // ragged band: per-lane window
     const uint4 rec = lr[x];
     const bool ok = x >= F.hw + d;
     const uint32_t c = ok ? score_at(smem, crs, r2s, rec, ok ? x - d : 0, sh, T2) : uint32_t(kOff);
     t2 = rf_step_rec(rec, t2, c);
-    vcol[size_t(x) * Dp] = hi16(t2);
+    pw[lane] = hi16(t2);
   }
   {
     // scores of the next 8 columns are computed while the chain runs the
@@ -203,7 +223,8 @@ __global__ __launch_bounds__(64) void k_row(BatchView b) {
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           t2 = rf_step_rec(lr[x + u], t2, c[u]);
-          vcol[size_t(x + u) * Dp] = hi16(t2);
+          pw[lane] = hi16(t2);
+          pw += Dp;
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) c[u] = cn[u];
@@ -211,137 +232,133 @@ __global__ __launch_bounds__(64) void k_row(BatchView b) {
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         t2 = rf_step_rec(lr[x + u], t2, c[u]);
-        vcol[size_t(x + u) * Dp] = hi16(t2);
+        pw[lane] = hi16(t2);
+        pw += Dp;
       }
       x += 8;
     }
-    for (; x < xend; ++x) {
+    for (; x < xend; ++x, pw += Dp) {
       const uint4 rec = lr[x];
       t2 = rf_step_rec(rec, t2, score_at(smem, crs, r2s, rec, x - d, sh, T2));
-      vcol[size_t(x) * Dp] = hi16(t2);
+      pw[lane] = hi16(t2);
     }
   }
-  for (int x = xend; x < W; ++x) {                       // right border: zero scores
+  for (int x = xend; x < W; ++x, pw += Dp) {             // right border: zero scores
     t2 = rf_step_rec(lr[x], t2, uint32_t(kOff));
-    vcol[size_t(x) * Dp] = hi16(t2);
+    pw[lane] = hi16(t2);
   }
 
   // ---- H-bwd: g'[W-1] = f'[W-1]; coefficient of x+1 ----
   int x = W - 2;
+  DE_GLOBAL uint16_t* pl = x0p + ptrdiff_t(x) * Dp;      // next forward value to load
+  DE_GLOBAL uint16_t* ps = pl;                           // next value to store
   uint32_t cur[U];
   // software pipeline: the U forward values of the next chunk are in flight
   // while the current chunk runs its chain
   const int xs = xa + U - 1;                             // chunks cover [xa, x]
   if (x >= xs) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) cur[u] = vcol[size_t(x - u) * Dp];
+    for (int u = 0; u < U; ++u) { cur[u] = pl[lane]; pl -= Dp; }
   }
   for (; x >= xs; x -= U) {
     uint32_t nxt[U];
     const bool more = x - U >= xs;
     if (more) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) nxt[u] = vcol[size_t(x - U - u) * Dp];
+      for (int u = 0; u < U; ++u) { nxt[u] = pl[lane]; pl -= Dp; }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       t2 = rf_step_rec(lr[x - u + 1], t2, cur[u]);
-      vcol[size_t(x - u) * Dp] = hi16(t2);
+      ps[lane] = hi16(t2);
+      ps -= Dp;
     }
     if (more) {
 #pragma unroll
       for (int u = 0; u < U; ++u) cur[u] = nxt[u];
     }
   }
-  for (; x >= xa; --x) {
-    const uint32_t c = vcol[size_t(x) * Dp];
+  for (; x >= xa; --x, ps -= Dp) {
+    const uint32_t c = ps[lane];
     t2 = rf_step_rec(lr[x + 1], t2, c);
-    vcol[size_t(x) * Dp] = hi16(t2);
+    ps[lane] = hi16(t2);
   }
-  for (; x >= 0; --x) {                                  // forward value is kOff here
+  for (; x >= 0; --x, ps -= Dp) {                        // forward value is kOff here
     t2 = rf_step_rec(lr[x + 1], t2, uint32_t(kOff));
-    vcol[size_t(x) * Dp] = hi16(t2);
+    ps[lane] = hi16(t2);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Stage 3: V-fwd + V-bwd + exact top-2 with sub-pixel neighbours, one wave per
-// column x holding all G disparity groups (G chains per lane).
+// Stage 3: V-fwd + V-bwd + exact top-2 with sub-pixel neighbours.
 //
-// V-bwd runs in blocks of 64 rows. While a block is filtered, every lane drops
-// its final values into an LDS block (row-major, groups packed two per word,
-// row stride padded by one word so the transposed reads below hit 64 distinct
-// banks). Then the block is read transposed: lane r owns row r of the block
-// and runs the top-2 over that row's disparities itself, on packed keys
-// ((v'+1) << 8 | 255-k), so ties resolve to the smaller k exactly as the
-// reference's ascending-k strictly-greater insertion. The neighbours k*-1 and
-// k*+1 are two more LDS reads. No cross-lane reduction is needed at all. The
-// V-bwd values never go back to memory; the output is one 16-byte record per
-// pixel.
+// A workgroup owns one column x: G waves, wave g holding disparity group g
+// (one chain per lane). V-bwd runs in blocks of 64 rows. While a block is
+// filtered, every lane drops its final value into an LDS block (row-major,
+// groups 2w and 2w+1 packed in one word, row stride padded by one word so the
+// transposed reads hit distinct banks). After a barrier the block is read
+// transposed: each row is owned by one lane, which runs the top-2 over that
+// row's disparities on packed keys ((v'+1) << 8 | 255-k). So ties resolve to
+// the smaller k, exactly as the reference's ascending-k strictly-greater
+// insertion does. The neighbours k*-1 and k*+1 are two more LDS reads. There
+// is no cross-lane reduction. The V-bwd values never go back to memory; the
+// output is one 16-byte record per pixel.
+//
+// The volume carries kVolPadRows rows of padding above and below the frame,
+// so the prefetch runs U..2U rows ahead without bounds checks (the padding
+// is read, never used).
 
-// U: rows prefetched ahead in both vertical passes (bytes in flight per wave =
-// U * 128 * G; a divisor of 64)
-template <int GM, int U>
-__global__ __launch_bounds__(64) void k_col(BatchView b) {
-  constexpr int NWL = (GM + 1) / 2;          // LDS words per lane per row
+template <int G, int U, bool STORE>
+__global__ __launch_bounds__(64 * G) void k_col(BatchView b, int base) {
+  constexpr int NWL = (G + 1) / 2;           // LDS words per lane per row
   constexpr int RS = kWave * NWL + 1;        // padded row stride (words)
-  static_assert(kWave % U == 0, "U must divide the 64-row block");
+  constexpr int RPW = (kWave + G - 1) / G;   // rows per wave in the transposed phase
+  static_assert(kWave % U == 0 && 2 * U <= kVolPadRows, "prefetch depth");
   __shared__ uint32_t rb[kWave * RS];
-  const int item = blockIdx.x;
+  const int item = blockIdx.x + base;
   const int f = find_frame(b, kStageCol, item);
   const FrameDesc F = b.frames[f];
   const int x = local_item(b, kStageCol, item, f);
-  const int W = F.W, H = F.H, Dp = F.Dpad, G = F.G;
-  const int lane = threadIdx.x;
+  const int W = F.W, H = F.H, Dp = F.Dpad;
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
+  const uint32_t lane = threadIdx.x & 63;
   const ptrdiff_t ys = ptrdiff_t(W) * Dp;                // row stride (elements)
-  DE_GLOBAL uint16_t* const v0 = gptr(F.vol) + size_t(x) * Dp + lane;   // row 0, group 0
+  // wave-uniform base of this column and group; lane l reads base[l]
+  DE_GLOBAL uint16_t* const col = gptr(F.vol) + size_t(x) * Dp + g * kWave;
   const DE_CONST uint32_t* ay = cptr(F.ayt) + size_t(x) * H;
-  DE_GLOBAL uint4* rec = gptr(reinterpret_cast<uint4*>(F.cand)) + x;   // row y at rec[y*W]
-  uint32_t t[GM];
-#pragma unroll
-  for (int g = 0; g < GM; ++g) t[g] = g < G ? uint32_t(v0[g * kWave]) << 16 : 0u;
+  uint16_t* const rb16 = reinterpret_cast<uint16_t*>(rb);
+  const int slot = 2 * ((g >> 1) * kWave + lane) + (g & 1);   // u16 index within a row
+  uint32_t t = uint32_t(col[lane]) << 16;
 
-  // rows [y0, y0 + dir*U) of the G groups at this lane's disparity
-  auto load_rows = [&](uint32_t (&dst)[U][GM], int y0, int dir) {
-    DE_GLOBAL uint16_t* p = v0 + ptrdiff_t(y0) * ys;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int yy = y0 + dir * u;
-#pragma unroll
-      for (int g = 0; g < GM; ++g)
-        dst[u][g] = (yy >= 0 && yy < H && g < G) ? uint32_t(p[g * kWave]) : 0u;
-      p += dir * ys;
-    }
-  };
-
-  // ---- V-fwd over rows 1..H-1, U rows in flight ----
+  // ---- V-fwd over rows 1..H-1 ----
   {
-    uint32_t cur[U][GM];
-    load_rows(cur, 1, 1);
-    DE_GLOBAL uint16_t* p = v0 + ys;
-    for (int y0 = 1; y0 < H; y0 += U) {
-      uint32_t nxt[U][GM];
-      const bool more = y0 + U < H;
-      if (more) load_rows(nxt, y0 + U, 1);
+    DE_GLOBAL uint16_t* pf = col + ys;                   // prefetch pointer
+    DE_GLOBAL uint16_t* ps = col + ys;                   // store pointer
+    uint32_t cur[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { cur[u] = pf[lane]; pf += ys; }
+    int y = 1;
+    for (; y + U <= H; y += U) {
+      uint32_t nxt[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) { nxt[u] = pf[lane]; pf += ys; }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        if (y0 + u < H) {
-          const uint32_t a2 = ay[y0 + u];
-#pragma unroll
-          for (int g = 0; g < GM; ++g) {
-            if (g < G) {
-              t[g] = rf_chain(a2, t[g], rf_acc(a2, cur[u][g]));
-              p[g * kWave] = hi16(t[g]);
-            }
-          }
-        }
-        p += ys;
+        const uint32_t a2 = ay[y + u];
+        t = rf_chain(a2, t, rf_acc(a2, cur[u]));
+        ps[lane] = hi16(t);
+        ps += ys;
       }
-      if (more) {
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+      for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+    }
 #pragma unroll
-          for (int g = 0; g < GM; ++g) cur[u][g] = nxt[u][g];
+    for (int u = 0; u < U; ++u) {                        // tail (< U rows)
+      if (y + u < H) {
+        const uint32_t a2 = ay[y + u];
+        t = rf_chain(a2, t, rf_acc(a2, cur[u]));
+        ps[lane] = hi16(t);
+        ps += ys;
       }
     }
   }
@@ -349,56 +366,51 @@ __global__ __launch_bounds__(64) void k_col(BatchView b) {
   // ---- V-bwd in blocks of 64 rows, transposed top-2 ----
   const int kmax = (x >= F.hw + F.dmin && x < W - F.hw) ? min(F.D - 1, x - F.hw - F.dmin) : -1;
   const int ylo = F.hh, yhi = H - F.hh;                  // rows with candidates
-  if (lane == 0) rec[size_t(H - 1) * W] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
-  uint32_t cur[U][GM];
-  load_rows(cur, H - 2, -1);
-  DE_GLOBAL uint16_t* p = v0 + ptrdiff_t(H - 2) * ys;
+  DE_GLOBAL uint4* rec = gptr(reinterpret_cast<uint4*>(F.cand)) + x;   // row y at rec[y*W]
+  if (threadIdx.x == 0) rec[size_t(H - 1) * W] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+  DE_GLOBAL uint16_t* pf = col + ptrdiff_t(H - 2) * ys;
+  DE_GLOBAL uint16_t* ps = pf;
+  uint32_t cur[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) { cur[u] = pf[lane]; pf -= ys; }
   for (int yb = H - 2; yb >= 0; yb -= kWave) {          // block: rows yb, yb-1, ..., yb-63
     const int nrow = min(kWave, yb + 1);
-    for (int u0 = 0; u0 < nrow; u0 += U) {
-      const int y0 = yb - u0;
-      uint32_t nxt[U][GM];
-      const bool more = y0 - U >= 0;
-      if (more) load_rows(nxt, y0 - U, -1);
+    int u0 = 0;
+    for (; u0 + U <= nrow; u0 += U) {
+      uint32_t nxt[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) { nxt[u] = pf[lane]; pf -= ys; }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int yy = y0 - u;
-        if (yy >= 0 && u0 + u < nrow) {
-          const uint32_t a2 = ay[yy + 1];
-          uint32_t v[GM];
-#pragma unroll
-          for (int g = 0; g < GM; ++g) {
-            if (g < G) {
-              t[g] = rf_chain(a2, t[g], rf_acc(a2, cur[u][g]));
-              v[g] = t[g] >> 16;
-              if (F.store_vol) p[g * kWave] = uint16_t(v[g]);
-            } else {
-              v[g] = 0u;
-            }
-          }
-#pragma unroll
-          for (int w = 0; w < NWL; ++w) {
-            const uint32_t lo = v[2 * w];
-            const uint32_t hi = (2 * w + 1 < GM) ? v[2 * w + 1] : 0u;
-            rb[(u0 + u) * RS + w * kWave + lane] = lo | (hi << 16);
-          }
-        }
-        p -= ys;
+        const uint32_t a2 = ay[yb - u0 - u + 1];
+        t = rf_chain(a2, t, rf_acc(a2, cur[u]));
+        const uint32_t v = t >> 16;
+        if (STORE) ps[lane] = uint16_t(v);
+        ps -= ys;
+        rb16[(u0 + u) * (2 * RS) + slot] = uint16_t(v);
       }
-      if (more) {
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+      for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+    }
 #pragma unroll
-          for (int g = 0; g < GM; ++g) cur[u][g] = nxt[u][g];
+    for (int u = 0; u < U; ++u) {                        // tail of the last block
+      if (u0 + u < nrow) {
+        const uint32_t a2 = ay[yb - u0 - u + 1];
+        t = rf_chain(a2, t, rf_acc(a2, cur[u]));
+        const uint32_t v = t >> 16;
+        if (STORE) ps[lane] = uint16_t(v);
+        ps -= ys;
+        rb16[(u0 + u) * (2 * RS) + slot] = uint16_t(v);
       }
     }
     __syncthreads();
-    // lane r: top-2 of row yb - r over k in [0, kmax]
-    if (lane < nrow) {
-      const int yy = yb - lane;
+    // transposed: wave g, lane r < RPW owns row yb - (g*RPW + r)
+    const int r = g * RPW + lane;
+    if (lane < RPW && r < nrow) {
+      const int yy = yb - r;
       uint32_t p0 = 0u, p1 = 0u, nbw = 0xFFFFFFFFu;
       if (yy >= ylo && yy < yhi && kmax >= 0) {
-        const uint32_t* row = rb + lane * RS;
+        const uint32_t* row = rb + r * RS;
         const int nw = min(NWL * kWave, (kmax >> 7) * kWave + kWave);
         for (int j = 0; j < nw; ++j) {
           const uint32_t wv = row[j];
@@ -471,7 +483,7 @@ __global__ __launch_bounds__(256) void k_census_plane(const uint8_t* __restrict_
 // Host launchers (called from the C-ABI translation unit).
 size_t row_lds_bytes(int W, int nbits, int sc_shift) {
   const int ntbl = (nbits + 1) << sc_shift;
-  return size_t((ntbl * 2 + 15) & ~15) + size_t(W) * 8 + size_t(W) * 2;
+  return size_t((ntbl * 2 + 15) & ~15) + size_t(W) * (16 + 8 + 2);
 }
 
 hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_t s) {
@@ -496,22 +508,29 @@ hipError_t launch_row(const BatchView& b, int items, size_t lds, int variant, hi
   return hipGetLastError();
 }
 
-template <int GM>
-static void launch_col_g(const BatchView& b, int items, int variant, hipStream_t s) {
+template <int G, bool STORE>
+static void launch_col_g(const BatchView& b, int base, int items, int variant, hipStream_t s) {
   switch (variant) {
-    case 0: hipLaunchKernelGGL((k_col<GM, 8>), dim3(items), dim3(64), 0, s, b); break;
-    case 1: hipLaunchKernelGGL((k_col<GM, 16>), dim3(items), dim3(64), 0, s, b); break;
-    default: hipLaunchKernelGGL((k_col<GM, 32>), dim3(items), dim3(64), 0, s, b); break;
+    case 0: hipLaunchKernelGGL((k_col<G, 8, STORE>), dim3(items), dim3(64 * G), 0, s, b, base); break;
+    case 1: hipLaunchKernelGGL((k_col<G, 16, STORE>), dim3(items), dim3(64 * G), 0, s, b, base); break;
+    default: hipLaunchKernelGGL((k_col<G, 32, STORE>), dim3(items), dim3(64 * G), 0, s, b, base); break;
   }
 }
 
-hipError_t launch_col(const BatchView& b, int items, int gmax, int variant, hipStream_t s) {
+// Column pass over the frames whose column items are [base, base + items);
+// every one of them must have exactly G disparity groups.
+hipError_t launch_col(const BatchView& b, int base, int items, int G, bool store, int variant,
+                      hipStream_t s) {
   if (variant < 0 || variant >= kNumColVariants) return hipErrorInvalidValue;
-  switch (gmax) {
-    case 1: launch_col_g<1>(b, items, variant, s); break;
-    case 2: launch_col_g<2>(b, items, variant, s); break;
-    case 3: launch_col_g<3>(b, items, variant, s); break;
-    case 4: launch_col_g<4>(b, items, variant, s); break;
+  switch (G * 2 + (store ? 1 : 0)) {
+    case 2: launch_col_g<1, false>(b, base, items, variant, s); break;
+    case 3: launch_col_g<1, true>(b, base, items, variant, s); break;
+    case 4: launch_col_g<2, false>(b, base, items, variant, s); break;
+    case 5: launch_col_g<2, true>(b, base, items, variant, s); break;
+    case 6: launch_col_g<3, false>(b, base, items, variant, s); break;
+    case 7: launch_col_g<3, true>(b, base, items, variant, s); break;
+    case 8: launch_col_g<4, false>(b, base, items, variant, s); break;
+    case 9: launch_col_g<4, true>(b, base, items, variant, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

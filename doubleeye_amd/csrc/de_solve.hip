@@ -31,7 +31,7 @@ namespace de {
 
 namespace {
 
-constexpr int kSolveThreads = 256;
+constexpr int kSolveThreads = 1024;
 
 __device__ __forceinline__ float fmax_ref(float a, float b) {   // std::max(a, b)
   return (a < b) ? b : a;
