@@ -151,6 +151,10 @@ def main():
     per = {n: stage_ms[i] / max(1, runs) for i, n in enumerate(names)}
     dom = max(per, key=per.get)
     ab = algorithmic_bytes(W, H, D)
+    # HBM bytes per launch of each kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # on this workload (profiles/dense_c2_848x480x128_kernels.md), scaled by the
+    # known-size copy/fill calibration (reads x2.00, writes x1.00)
+    traffic = {"row": 306.9e6, "col": 309.1e6, "solve": 9.9e6, "prep": 15.5e6}
     achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
     frame_gbs = ab["frame"] / (ms * 1e-3) / 1e9
     out = {
@@ -168,7 +172,7 @@ def main():
                    "W": W, "H": H, "D": D, "iters": 8},
         "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
                      "kernel_ms": round(per[dom], 4),
                      "stage_ms": {n: round(v, 4) for n, v in per.items()},
                      "frame_algorithmic_bytes": ab["frame"],
