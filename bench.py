@@ -103,48 +103,51 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     from doubleeye_amd import synth
     from doubleeye_amd.dense import Cfg, Context
 
+    # Device memory and streams come from the library's own HIP runtime; no
+    # second runtime (e.g. torch's bundled one) is loaded into the process.
     L, R, _ = synth.slanted_pair(W, H, seed=2)
-    dev = torch.device("cuda:0")
+    ctx = Context(0)
     if a.calibrate_copy:
         n = a.calibrate_copy << 20
-        src = torch.ones(n, dtype=torch.uint8, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-    dL = torch.from_numpy(L).to(dev)
-    dR = torch.from_numpy(R).to(dev)
-    disp = torch.empty((H, W), dtype=torch.float32, device=dev)
-    margin = torch.empty((H, W), dtype=torch.float32, device=dev)
-    ctx = Context(0)
+        src = ctx.to_device(np.ones(n, np.uint8))
+        dst = ctx.empty((n,), np.uint8)
+        ctx.copy_dev(dst, src)          # one plain device-to-device copy of n bytes
+        ctx.fill_dev(dst, 0)            # one plain device fill of n bytes
+        ctx.sync()
+        src.free(); dst.free()
+    dL, dR = ctx.to_device(L), ctx.to_device(R)
+    disp = ctx.empty((H, W), np.float32)
+    margin = ctx.empty((H, W), np.float32)
     cfg = Cfg(**CFG)
-    stream = torch.cuda.current_stream(dev).cuda_stream
 
     def step():
-        ctx.run_dense_dev(dL.data_ptr(), dR.data_ptr(), W, H, cfg, disp.data_ptr(),
-                          margin.data_ptr(), 0, stream)
+        ctx.run_dense_dev(dL.ptr, dR.ptr, W, H, cfg, disp.ptr, margin.ptr)
 
+    step()
+    ctx.sync()
+    first = disp.numpy()
     for _ in range(a.warmup):
         step()
-    torch.cuda.synchronize()
+    ctx.sync()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         step()
-    torch.cuda.synchronize()
+    ctx.sync()
     dt = time.perf_counter() - t0
     ms = dt * 1e3 / a.steps
     value = W * H * D / (ms * 1e-3) / 1e6
+    last = disp.numpy()
+    deterministic = bool(np.array_equal(first.view(np.uint32), last.view(np.uint32)))
 
     # per-stage kernel durations, hipEvents between launches on this stream
     ctx.set_profiling(True)
     for _ in range(max(20, a.steps // 4)):
         step()
-    torch.cuda.synchronize()
+    ctx.sync()
     stage_ms, runs = ctx.stage_ms()
     ctx.set_profiling(False)
     names = ["prep", "row", "col", "solve"]
@@ -164,6 +167,8 @@ def main():
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(ms, 4),
         "frames_per_s": round(1e3 / ms, 1),
+        "coverage": round(float(np.isfinite(last).mean()), 4),
+        "deterministic": deterministic,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "i16 volume / f32 messages",
         "data": "synthetic (seeded slanted-plane pair, doubleeye_amd.synth.slanted_pair seed 2)",

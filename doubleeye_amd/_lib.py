@@ -37,6 +37,13 @@ class DeCfgC(C.Structure):
                 ("subpixel", C.c_int), ("fit_eq", C.c_int), ("lrc", C.c_int)]
 
 
+class DePairC(C.Structure):
+    """de_pair (include/doubleeye_hip.h): device pointers of one batch pair."""
+    _fields_ = [("L", C.c_void_p), ("R", C.c_void_p), ("W", C.c_int), ("H", C.c_int),
+                ("disp", C.c_void_p), ("margin", C.c_void_p), ("cand6", C.c_void_p),
+                ("conf", C.c_void_p), ("q4", C.c_void_p)]
+
+
 # (name, restype, argtypes) for every exported entry point
 P = C.c_void_p
 I = C.c_int
@@ -48,6 +55,12 @@ SIGNATURES = {
     "de_ctx_sync": (I, [P]),
     "de_ctx_stream": (P, [P]),
     "de_last_error": (C.c_char_p, []),
+    "de_alloc": (I, [P, C.c_size_t, C.POINTER(P)]),
+    "de_free": (I, [P, P]),
+    "de_upload": (I, [P, P, P, C.c_size_t]),
+    "de_download": (I, [P, P, P, C.c_size_t]),
+    "de_copy_dev": (I, [P, P, P, C.c_size_t]),
+    "de_fill_dev": (I, [P, P, I, C.c_size_t]),
     "de_census": (I, [P, P, I, I, I, I, P]),
     "de_census_dev": (I, [P, P, I, I, I, I, P, P]),
     "de_dense_run": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, P, P]),
@@ -57,6 +70,11 @@ SIGNATURES = {
     "de_ctx_set_profiling": (I, [P, I]),
     "de_ctx_set_variants": (I, [P, I, I]),
     "de_ctx_stage_ms": (I, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "de_dense_batch_dev": (I, [P, C.POINTER(DePairC), C.POINTER(DeCfgC), I, P]),
+    "de_plan_create": (I, [P, C.POINTER(DePairC), C.POINTER(DeCfgC), I, C.POINTER(P)]),
+    "de_plan_launch": (I, [P, P]),
+    "de_plan_destroy": (None, [P]),
+    "de_autotune": (I, [P, I, I, C.POINTER(DeCfgC), I, C.POINTER(I), C.POINTER(I)]),
 }
 
 _lib = None

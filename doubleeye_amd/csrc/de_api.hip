@@ -215,82 +215,102 @@ Scratch carve_scratch(void* base, int W, int H, int D) {
   return s;
 }
 
-// Launch every stage over `n` frames (stage_mask selects stages, for tests).
-int Ctx::run_frames(const FrameDesc* fds_in, int n, hipStream_t s, int stage_mask) {
+void Ctx::Prepared::pack(void* dst) const {
+  std::memcpy(dst, frames.data(), frames.size() * sizeof(FrameDesc));
+  std::memcpy(static_cast<uint8_t*>(dst) + frames.size() * sizeof(FrameDesc), start.data(),
+              start.size() * sizeof(int));
+}
+
+// Order the frames by disparity-group count (each column-pass bucket, one
+// kernel instantiation per G, is then a contiguous range of work items),
+// build the per-stage work prefixes and pick the launch variants.
+int Ctx::prepare(const FrameDesc* fds_in, int n, Prepared* P) const {
   if (n < 1) return fail(DE_ERR_ARG, "empty batch");
-  // frames ordered by disparity-group count, so each column-pass bucket (one
-  // kernel instantiation per G) is a contiguous range of work items
-  std::vector<FrameDesc> sorted(fds_in, fds_in + n);
-  std::stable_sort(sorted.begin(), sorted.end(),
+  P->frames.assign(fds_in, fds_in + n);
+  std::stable_sort(P->frames.begin(), P->frames.end(),
                    [](const FrameDesc& a, const FrameDesc& b) { return a.G < b.G; });
-  const FrameDesc* fds = sorted.data();
-  std::vector<int> start(size_t(kNumStages) * (n + 1));
-  int gmax = 1;
-  size_t lds_row = 0, lds_solve = 0;
-  const int hw = fds[0].hw, hh = fds[0].hh;
+  const FrameDesc* fds = P->frames.data();
+  P->start.assign(size_t(kNumStages) * (n + 1), 0);
   for (int st = 0; st < kNumStages; ++st) {
     int acc = 0;
     for (int i = 0; i < n; ++i) {
-      start[size_t(st) * (n + 1) + i] = acc;
+      P->start[size_t(st) * (n + 1) + i] = acc;
       acc += stage_items(fds[i], st);
     }
-    start[size_t(st) * (n + 1) + n] = acc;
+    P->start[size_t(st) * (n + 1) + n] = acc;
   }
+  const int hw = fds[0].hw, hh = fds[0].hh;
+  int big = 0;
+  P->lds_row = P->lds_solve = 0;
   for (int i = 0; i < n; ++i) {
     if (fds[i].hw != hw || fds[i].hh != hh)
       return fail(DE_ERR_ARG, "a batch must share one census window");
-    gmax = std::max(gmax, fds[i].G);
     const int nbits = (2 * fds[i].hw + 1) * (2 * fds[i].hh + 1) - 1;
-    lds_row = std::max(lds_row, row_lds_bytes(fds[i].W, nbits, fds[i].sc_shift));
-    lds_solve = std::max(lds_solve, solve_lds_bytes(fds[i].W));
+    P->lds_row = std::max(P->lds_row, row_lds_bytes(fds[i].W, nbits, fds[i].sc_shift));
+    P->lds_solve = std::max(P->lds_solve, solve_lds_bytes(fds[i].W));
+    if (size_t(fds[i].W) * fds[i].H * fds[i].D > size_t(fds[big].W) * fds[big].H * fds[big].D)
+      big = i;
   }
-  if (lds_row > 160 * 1024 || lds_solve > 160 * 1024)
-    return fail(DE_ERR_ARG, "row too wide for LDS (row %zu, solve %zu bytes)", lds_row, lds_solve);
-  const size_t bytes = sizeof(FrameDesc) * n + start.size() * sizeof(int);
+  if (P->lds_row > 160 * 1024 || P->lds_solve > 160 * 1024)
+    return fail(DE_ERR_ARG, "row too wide for LDS (row %zu, solve %zu bytes)", P->lds_row,
+                P->lds_solve);
+  // the variants of the largest frame's shape bucket drive the launch
+  P->rowv = pick_row_variant(fds[big].W, fds[big].H, fds[big].G);
+  P->colv = pick_col_variant(fds[big].W, fds[big].H, fds[big].G);
+  return DE_OK;
+}
+
+int Ctx::launch_stages(const Prepared& P, const BatchView& bv, hipStream_t s, int stage_mask,
+                       EvSet* ev) {
+  const int n = int(P.frames.size());
+  const FrameDesc* fds = P.frames.data();
+  auto items = [&](int st) { return P.start[size_t(st) * (n + 1) + n]; };
+  hipError_t e;
+  if (ev) DE_HIP(hipEventRecord(ev->e[0], s));
+  if ((stage_mask & 1) && (e = launch_prep(bv, items(kStagePrep), fds[0].hw, fds[0].hh, s)) != hipSuccess)
+    return hip_fail(e, "k_prep");
+  if (ev) DE_HIP(hipEventRecord(ev->e[1], s));
+  if ((stage_mask & 2) && (e = launch_row(bv, items(kStageRow), P.lds_row, P.rowv, s)) != hipSuccess)
+    return hip_fail(e, "k_row");
+  if (ev) DE_HIP(hipEventRecord(ev->e[2], s));
+  if (stage_mask & 4) {
+    for (int i = 0; i < n;) {
+      int j = i;
+      while (j < n && fds[j].G == fds[i].G) ++j;
+      const int base = P.start[size_t(kStageCol) * (n + 1) + i];
+      const int cnt = P.start[size_t(kStageCol) * (n + 1) + j] - base;
+      if ((e = launch_col(bv, base, cnt, fds[i].G, fds[i].store_vol != 0, P.colv, s)) != hipSuccess)
+        return hip_fail(e, "k_col");
+      i = j;
+    }
+  }
+  if (ev) DE_HIP(hipEventRecord(ev->e[3], s));
+  if ((stage_mask & 8) && (e = launch_solve(bv, items(kStageSolve), P.lds_solve, s)) != hipSuccess)
+    return hip_fail(e, "k_solve");
+  if (ev) DE_HIP(hipEventRecord(ev->e[4], s));
+  return DE_OK;
+}
+
+// Launch every stage over `n` frames (stage_mask selects stages, for tests).
+int Ctx::run_frames(const FrameDesc* fds_in, int n, hipStream_t s, int stage_mask) {
+  Prepared P;
+  if (int rc = prepare(fds_in, n, &P)) return rc;
+  const size_t bytes = P.desc_bytes();
   Slot* slot;
   if (int rc = acquire_slot(bytes, &slot)) return rc;
-  std::memcpy(slot->host, fds, sizeof(FrameDesc) * n);
-  std::memcpy(static_cast<uint8_t*>(slot->host) + sizeof(FrameDesc) * n, start.data(),
-              start.size() * sizeof(int));
+  P.pack(slot->host);
   DE_HIP(hipMemcpyAsync(slot->dev, slot->host, bytes, hipMemcpyHostToDevice, s));
   BatchView bv;
   bv.frames = static_cast<const FrameDesc*>(slot->dev);
   bv.start = reinterpret_cast<const int*>(static_cast<uint8_t*>(slot->dev) + sizeof(FrameDesc) * n);
   bv.n = n;
-  auto items = [&](int st) { return start[size_t(st) * (n + 1) + n]; };
   const bool prof = profiling && stage_mask == 0xF;
   EvSet ev;
   if (prof) {
     if (int rc = take_evset(&ev)) return rc;
-    DE_HIP(hipEventRecord(ev.e[0], s));
   }
-  hipError_t e;
-  if ((stage_mask & 1) && (e = launch_prep(bv, items(kStagePrep), hw, hh, s)) != hipSuccess)
-    return hip_fail(e, "k_prep");
-  if (prof) DE_HIP(hipEventRecord(ev.e[1], s));
-  const int rowv = pick_row_variant(fds[0].W, fds[0].H, gmax);
-  const int colv = pick_col_variant(fds[0].W, fds[0].H, gmax);
-  if ((stage_mask & 2) && (e = launch_row(bv, items(kStageRow), lds_row, rowv, s)) != hipSuccess)
-    return hip_fail(e, "k_row");
-  if (prof) DE_HIP(hipEventRecord(ev.e[2], s));
-  if (stage_mask & 4) {
-    for (int i = 0; i < n;) {
-      int j = i;
-      while (j < n && fds[j].G == fds[i].G) ++j;
-      const int base = start[size_t(kStageCol) * (n + 1) + i];
-      const int cnt = start[size_t(kStageCol) * (n + 1) + j] - base;
-      if ((e = launch_col(bv, base, cnt, fds[i].G, fds[i].store_vol != 0, colv, s)) != hipSuccess)
-        return hip_fail(e, "k_col");
-      i = j;
-    }
-  }
-  if (prof) DE_HIP(hipEventRecord(ev.e[3], s));
-  if ((stage_mask & 8) && (e = launch_solve(bv, items(kStageSolve), lds_solve, s)) != hipSuccess)
-    return hip_fail(e, "k_solve");
-  if (prof) {
-    DE_HIP(hipEventRecord(ev.e[4], s));
-    ev_pending.push_back(ev);
-  }
+  if (int rc = launch_stages(P, bv, s, stage_mask, prof ? &ev : nullptr)) return rc;
+  if (prof) ev_pending.push_back(ev);
   DE_HIP(hipEventRecord(slot->done, s));
   slot->used = true;
   return DE_OK;
@@ -310,13 +330,15 @@ int Ctx::dense_dev(const uint8_t* dL, const uint8_t* dR, int W, int H, const de_
 
 // Defaults until the autotuner has measured a shape: 16 rows in flight in the
 // column pass, 32 columns in the row pass's backward scan.
-int Ctx::pick_row_variant(int, int, int) const {
+int Ctx::pick_row_variant(int W, int H, int G) const {
   if (force_row_variant >= 0) return force_row_variant;
-  return 2;
+  const auto it = tuned.find(std::make_tuple(W, H, G));
+  return it != tuned.end() ? it->second.first : 2;
 }
-int Ctx::pick_col_variant(int, int, int G) const {
+int Ctx::pick_col_variant(int W, int H, int G) const {
   if (force_col_variant >= 0) return force_col_variant;
-  return 1;
+  const auto it = tuned.find(std::make_tuple(W, H, G));
+  return it != tuned.end() ? it->second.second : 1;
 }
 
 int Ctx::take_evset(EvSet* out) {
@@ -440,6 +462,69 @@ int de_ctx_sync(de_ctx* c) {
   return DE_OK;
 }
 
+int de_alloc(de_ctx* c, size_t bytes, void** d_ptr) {
+  if (!c || !d_ptr) return fail(DE_ERR_ARG, "null argument");
+  *d_ptr = nullptr;
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  DE_HIP(hipSetDevice(c->c.device));
+  DE_HIP(hipMalloc(d_ptr, std::max(bytes, size_t(1))));
+  return DE_OK;
+}
+
+int de_free(de_ctx* c, void* d_ptr) {
+  if (!c) return fail(DE_ERR_ARG, "null ctx");
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  DE_HIP(hipSetDevice(c->c.device));
+  if (d_ptr) DE_HIP(hipFree(d_ptr));
+  return DE_OK;
+}
+
+int de_upload(de_ctx* c, void* d_dst, const void* h_src, size_t bytes) {
+  if (!c || (!d_dst && bytes) || (!h_src && bytes)) return fail(DE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  DE_HIP(hipSetDevice(c->c.device));
+  DE_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, c->c.stream));
+  DE_HIP(hipStreamSynchronize(c->c.stream));
+  return DE_OK;
+}
+
+int de_download(de_ctx* c, void* h_dst, const void* d_src, size_t bytes) {
+  if (!c || (!d_src && bytes) || (!h_dst && bytes)) return fail(DE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  DE_HIP(hipSetDevice(c->c.device));
+  DE_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, c->c.stream));
+  DE_HIP(hipStreamSynchronize(c->c.stream));
+  return DE_OK;
+}
+
+int de_copy_dev(de_ctx* c, void* d_dst, const void* d_src, size_t bytes) {
+  if (!c || !d_dst || !d_src) return fail(DE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  DE_HIP(hipSetDevice(c->c.device));
+  DE_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, c->c.stream));
+  return DE_OK;
+}
+
+// A fill kernel rather than hipMemsetAsync, so the same call is safe inside a
+// captured graph.
+__global__ void k_fill_bytes(uint8_t* __restrict__ p, uint8_t v, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+int de_fill_dev(de_ctx* c, void* d_dst, int byte, size_t bytes) {
+  if (!c || !d_dst) return fail(DE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  DE_HIP(hipSetDevice(c->c.device));
+  const size_t blocks = std::min<size_t>(65536, (bytes + 255) / 256);
+  if (blocks)
+    hipLaunchKernelGGL(k_fill_bytes, dim3(unsigned(blocks)), dim3(256), 0, c->c.stream,
+                       static_cast<uint8_t*>(d_dst), uint8_t(byte), bytes);
+  DE_HIP(hipGetLastError());
+  return DE_OK;
+}
+
 int de_census_dev(de_ctx* c, const uint8_t* d_img, int W, int H, int hw, int hh,
                   uint64_t* d_out, void* stream) {
   if (!c || !d_img || !d_out || W < 1 || H < 1) return fail(DE_ERR_ARG, "de_census_dev args");
@@ -516,6 +601,230 @@ int de_dense_run(de_ctx* c, const uint8_t* L, const uint8_t* R, int W, int H,
   if (!c) return fail(DE_ERR_ARG, "null ctx");
   std::lock_guard<std::mutex> lk(c->c.mu);
   return dense_host(c, L, R, W, H, cfg, disp, margin, cand6, 0xF);
+}
+
+// ---- batches, plans, autotuner ----
+
+static int batch_check(const de_pair* pairs, const de_cfg* cfgs, int n) {
+  if (!pairs || !cfgs || n < 1) return fail(DE_ERR_ARG, "batch: %d pairs", n);
+  for (int i = 0; i < n; ++i) {
+    const de_pair& p = pairs[i];
+    if (!p.L || !p.R || !p.disp || !p.margin)
+      return fail(DE_ERR_ARG, "batch pair %d: null input or output", i);
+    if (int rc = validate(cfgs[i], p.W, p.H)) return rc;
+    const de_cfg& a = cfgs[0];
+    const de_cfg& b = cfgs[i];
+    if (a.census_hw != b.census_hw || a.census_hh != b.census_hh || a.sigma_s != b.sigma_s ||
+        a.sigma_r != b.sigma_r || a.ad != b.ad || a.ad_trunc != b.ad_trunc)
+      return fail(DE_ERR_ARG,
+                  "batch pair %d: census window, sigma_s, sigma_r, ad and ad_trunc must match "
+                  "the batch's first pair", i);
+  }
+  return DE_OK;
+}
+
+static size_t batch_scratch_bytes(const de_pair* pairs, const de_cfg* cfgs, int n,
+                                  std::vector<size_t>* offs) {
+  size_t total = 0;
+  offs->assign(size_t(n), 0);
+  for (int i = 0; i < n; ++i) {
+    (*offs)[i] = total;
+    total += (scratch_bytes(pairs[i].W, pairs[i].H, cfgs[i].dmax - cfgs[i].dmin + 1) + 255) &
+             ~size_t(255);
+  }
+  return total;
+}
+
+int de_dense_batch_dev(de_ctx* c, const de_pair* pairs, const de_cfg* cfgs, int n,
+                       void* stream) {
+  if (!c) return fail(DE_ERR_ARG, "null ctx");
+  if (int rc = batch_check(pairs, cfgs, n)) return rc;
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  Ctx& x = c->c;
+  DE_HIP(hipSetDevice(x.device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : x.stream;
+  std::vector<size_t> offs;
+  const size_t total = batch_scratch_bytes(pairs, cfgs, n, &offs);
+  if (int rc = x.scratch.ensure(total)) return rc;
+  std::vector<FrameDesc> fds(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) {
+    const de_pair& p = pairs[i];
+    const Scratch sp = carve_scratch(static_cast<uint8_t*>(x.scratch.ptr) + offs[i], p.W, p.H,
+                                     cfgs[i].dmax - cfgs[i].dmin + 1);
+    if (int rc = x.frame_desc(cfgs[i], p.W, p.H, p.L, p.R, p.disp, p.margin, p.cand6, p.conf,
+                              p.q4, sp, &fds[i]))
+      return rc;
+  }
+  return x.run_frames(fds.data(), n, s, 0xF);
+}
+
+struct de_plan {
+  de_ctx* ctx = nullptr;
+  int device = 0;
+  DevBuf scratch, desc, tables;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+void de_plan_destroy(de_plan* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->device);
+  (void)hipDeviceSynchronize();
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->scratch.release();
+  p->desc.release();
+  p->tables.release();
+  delete p;
+}
+
+int de_plan_create(de_ctx* c, const de_pair* pairs, const de_cfg* cfgs, int n, de_plan** out) {
+  if (!c || !out) return fail(DE_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (int rc = batch_check(pairs, cfgs, n)) return rc;
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  Ctx& x = c->c;
+  DE_HIP(hipSetDevice(x.device));
+  de_plan* p = new de_plan();
+  p->ctx = c;
+  p->device = x.device;
+  auto bail = [&](int rc) { de_plan_destroy(p); return rc; };
+  // private tables: later configuration changes on the context do not touch them
+  const de_cfg& c0 = cfgs[0];
+  const int nbits = (2 * c0.census_hw + 1) * (2 * c0.census_hh + 1) - 1;
+  uint16_t rft[256];
+  build_rf_table(c0.sigma_s, c0.sigma_r, rft);
+  std::vector<uint16_t> sct;
+  build_score_table(nbits, c0.ad, c0.ad_trunc, &sct);
+  if (int rc = p->tables.ensure(512 + sct.size() * 2)) return bail(rc);
+  hipError_t e = hipMemcpy(p->tables.ptr, rft, 512, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(static_cast<uint8_t*>(p->tables.ptr) + 512, sct.data(), sct.size() * 2,
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) return bail(hip_fail(e, "plan tables"));
+  std::vector<size_t> offs;
+  const size_t total = batch_scratch_bytes(pairs, cfgs, n, &offs);
+  if (int rc = p->scratch.ensure(total)) return bail(rc);
+  std::vector<FrameDesc> fds(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) {
+    const de_pair& q = pairs[i];
+    const Scratch sp = carve_scratch(static_cast<uint8_t*>(p->scratch.ptr) + offs[i], q.W, q.H,
+                                     cfgs[i].dmax - cfgs[i].dmin + 1);
+    if (int rc = x.frame_desc(cfgs[i], q.W, q.H, q.L, q.R, q.disp, q.margin, q.cand6, q.conf,
+                              q.q4, sp, &fds[i]))
+      return bail(rc);
+    fds[i].rf = static_cast<const uint16_t*>(p->tables.ptr);
+    fds[i].sc = reinterpret_cast<const uint16_t*>(static_cast<uint8_t*>(p->tables.ptr) + 512);
+  }
+  Ctx::Prepared P;
+  if (int rc = x.prepare(fds.data(), n, &P)) return bail(rc);
+  if (int rc = p->desc.ensure(P.desc_bytes())) return bail(rc);
+  std::vector<uint8_t> host(P.desc_bytes());
+  P.pack(host.data());
+  e = hipMemcpy(p->desc.ptr, host.data(), host.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return bail(hip_fail(e, "plan descriptors"));
+  BatchView bv;
+  bv.frames = static_cast<const FrameDesc*>(p->desc.ptr);
+  bv.start = reinterpret_cast<const int*>(static_cast<uint8_t*>(p->desc.ptr) +
+                                          sizeof(FrameDesc) * n);
+  bv.n = n;
+  hipStream_t cs = nullptr;
+  e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(hip_fail(e, "plan capture stream"));
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  int rc = DE_OK;
+  if (e == hipSuccess) {
+    rc = x.launch_stages(P, bv, cs, 0xF, nullptr);
+    const hipError_t e2 = hipStreamEndCapture(cs, &p->graph);
+    if (rc == DE_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
+  } else {
+    rc = hip_fail(e, "hipStreamBeginCapture");
+  }
+  (void)hipStreamDestroy(cs);
+  if (rc != DE_OK) return bail(rc);
+  e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return bail(hip_fail(e, "hipGraphInstantiate"));
+  *out = p;
+  return DE_OK;
+}
+
+int de_plan_launch(de_plan* p, void* stream) {
+  if (!p || !p->exec) return fail(DE_ERR_ARG, "null plan");
+  DE_HIP(hipSetDevice(p->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->ctx->c.stream;
+  DE_HIP(hipGraphLaunch(p->exec, s));
+  return DE_OK;
+}
+
+int de_autotune(de_ctx* c, int W, int H, const de_cfg* cfg, int reps, int* row_variant,
+                int* col_variant) {
+  if (!c || !cfg) return fail(DE_ERR_ARG, "null argument");
+  if (int rc = validate(*cfg, W, H)) return rc;
+  reps = std::max(1, reps);
+  // synthetic textured pair (LCG bytes, right = left shifted by D/3)
+  const size_t WH = size_t(W) * H;
+  std::vector<uint8_t> hl(WH), hr(WH);
+  uint32_t st = 12345u;
+  for (size_t i = 0; i < WH; ++i) { st = st * 1664525u + 1013904223u; hl[i] = uint8_t(st >> 24); }
+  const int sh = std::max(1, (cfg->dmax - cfg->dmin + 1) / 3);
+  for (int y = 0; y < H; ++y)
+    for (int xx = 0; xx < W; ++xx)
+      hr[size_t(y) * W + xx] = hl[size_t(y) * W + std::min(W - 1, xx + sh)];
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  Ctx& x = c->c;
+  DE_HIP(hipSetDevice(x.device));
+  DevBuf io;
+  if (int rc = io.ensure(WH * 2 + WH * 8)) return rc;
+  uint8_t* dL = static_cast<uint8_t*>(io.ptr);
+  uint8_t* dR = dL + WH;
+  float* dd = reinterpret_cast<float*>(dR + WH);
+  float* dm = dd + WH;
+  int rc = DE_OK;
+  hipError_t e = hipMemcpy(dL, hl.data(), WH, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dR, hr.data(), WH, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { io.release(); return hip_fail(e, "autotune upload"); }
+  const bool prof0 = x.profiling;
+  const int fr0 = x.force_row_variant, fc0 = x.force_col_variant;
+  auto time_stage = [&](int stage, int rv, int cv, double* ms) -> int {
+    x.force_row_variant = rv;
+    x.force_col_variant = cv;
+    if (int r = x.dense_dev(dL, dR, W, H, *cfg, dd, dm, nullptr, x.stream, 0xF)) return r;  // warm
+    if (int r = x.drain_events()) return r;
+    x.profiling = true;
+    for (double& v : x.stage_ms) v = 0.0;
+    x.stage_runs = 0;
+    for (int i = 0; i < reps; ++i)
+      if (int r = x.dense_dev(dL, dR, W, H, *cfg, dd, dm, nullptr, x.stream, 0xF)) return r;
+    if (int r = x.drain_events()) return r;
+    x.profiling = false;
+    *ms = x.stage_ms[stage] / std::max(1, x.stage_runs);
+    return DE_OK;
+  };
+  double best_r = 1e30, best_c = 1e30;
+  int br = 2, bc = 1;
+  for (int v = 0; v < kNumRowVariants && rc == DE_OK; ++v) {
+    double ms = 0;
+    rc = time_stage(kStageRow, v, 1, &ms);
+    if (rc == DE_OK && ms < best_r) { best_r = ms; br = v; }
+  }
+  for (int v = 0; v < kNumColVariants && rc == DE_OK; ++v) {
+    double ms = 0;
+    rc = time_stage(kStageCol, br, v, &ms);
+    if (rc == DE_OK && ms < best_c) { best_c = ms; bc = v; }
+  }
+  x.profiling = prof0;
+  for (double& v : x.stage_ms) v = 0.0;
+  x.stage_runs = 0;
+  x.force_row_variant = fr0;
+  x.force_col_variant = fc0;
+  (void)hipStreamSynchronize(x.stream);
+  io.release();
+  if (rc != DE_OK) return rc;
+  const int G = (cfg->dmax - cfg->dmin + 1 + kWave - 1) / kWave;
+  x.tuned[std::make_tuple(W, H, G)] = std::make_pair(br, bc);
+  if (row_variant) *row_variant = br;
+  if (col_variant) *col_variant = bc;
+  return DE_OK;
 }
 
 int de_dense_volume(de_ctx* c, const uint8_t* L, const uint8_t* R, int W, int H,

@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "../../include/doubleeye_hip.h"
@@ -56,6 +58,8 @@ struct Ctx {
   int force_row_variant = -1, force_col_variant = -1;
   int pick_row_variant(int W, int H, int G) const;
   int pick_col_variant(int W, int H, int G) const;
+  // autotuned variants per shape bucket (W, H, G)
+  std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;
   // optional per-stage timing: hipEvents recorded between the stage launches
   struct EvSet { hipEvent_t e[kNumStages + 1]; };
   bool profiling = false;
@@ -75,6 +79,19 @@ struct Ctx {
   int frame_desc(const de_cfg& c, int W, int H, const uint8_t* L, const uint8_t* R,
                  float* disp, float* margin, float* cand6, uint8_t* conf, int16_t* q4,
                  const Scratch& sp, FrameDesc* fd);
+  // A batch ready to launch: frames ordered by G, per-stage work prefixes,
+  // launch geometry and the chosen variants.
+  struct Prepared {
+    std::vector<FrameDesc> frames;
+    std::vector<int> start;
+    size_t lds_row = 0, lds_solve = 0;
+    int rowv = 0, colv = 0;
+    size_t desc_bytes() const { return frames.size() * sizeof(FrameDesc) + start.size() * sizeof(int); }
+    void pack(void* dst) const;
+  };
+  int prepare(const FrameDesc* fds, int n, Prepared* out) const;
+  int launch_stages(const Prepared& p, const BatchView& bv, hipStream_t s, int stage_mask,
+                    EvSet* ev);
   int run_frames(const FrameDesc* fds, int n, hipStream_t s, int stage_mask);
   int dense_dev(const uint8_t* dL, const uint8_t* dR, int W, int H, const de_cfg& c,
                 float* disp, float* margin, float* cand6, hipStream_t s, int stage_mask);

@@ -18,9 +18,10 @@ from dataclasses import dataclass, fields
 
 import numpy as np
 
-from ._lib import DeCfgC, check, lib
+from ._lib import DeCfgC, DePairC, check, lib
 
-__all__ = ["Cfg", "Context", "run_dense", "census", "default_context"]
+__all__ = ["Cfg", "Context", "DeviceArray", "Pair", "Plan", "run_dense", "census",
+           "default_context"]
 
 
 @dataclass
@@ -54,6 +55,103 @@ class Cfg:
             v = getattr(self, f.name)
             setattr(c, name, int(v) if isinstance(v, bool) else v)
         return c
+
+
+class DeviceArray:
+    """Device memory owned through the library's HIP runtime (de_alloc), with a
+    numpy dtype/shape for uploads and downloads."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        h = C.c_void_p()
+        check(lib().de_alloc(ctx.h, self.nbytes, C.byref(h)))
+        self.ptr = int(h.value or 0)
+
+    @classmethod
+    def from_numpy(cls, ctx: "Context", a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"upload of {a.nbytes} bytes into {self.nbytes}")
+        check(lib().de_upload(self.ctx.h, C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p),
+                              self.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        check(lib().de_download(self.ctx.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            lib().de_free(self.ctx.h, C.c_void_p(self.ptr))
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class Pair:
+    """One pair of a batch: device addresses (ints) of the inputs and outputs."""
+    L: int
+    R: int
+    W: int
+    H: int
+    disp: int
+    margin: int
+    cand6: int = 0
+    conf: int = 0
+    q4: int = 0
+
+    def to_c(self) -> DePairC:
+        return DePairC(self.L, self.R, self.W, self.H, self.disp, self.margin,
+                       self.cand6 or None, self.conf or None, self.q4 or None)
+
+
+def _pairs_cfgs(pairs, cfgs):
+    n = len(pairs)
+    if len(cfgs) != n:
+        raise ValueError(f"{n} pairs but {len(cfgs)} configurations")
+    P = (DePairC * n)(*[p.to_c() for p in pairs])
+    Cs = (DeCfgC * n)(*[c.to_c() for c in cfgs])
+    return P, Cs, n
+
+
+class Plan:
+    """A captured batch (de_plan): resident scratch + a hipGraph of the stages."""
+
+    def __init__(self, ctx: "Context", pairs, cfgs):
+        P, Cs, n = _pairs_cfgs(pairs, cfgs)
+        h = C.c_void_p()
+        check(lib().de_plan_create(ctx.h, P, Cs, n, C.byref(h)))
+        self.h = h
+        self.n = n
+
+    def launch(self, stream: int = 0):
+        check(lib().de_plan_launch(self.h, C.c_void_p(stream) if stream else None))
+
+    def close(self):
+        if self.h:
+            lib().de_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _p(a: np.ndarray):
@@ -141,6 +239,33 @@ class Context:
                    stream: int = 0):
         check(lib().de_census_dev(self.h, C.c_void_p(d_img), W, H, hw, hh, C.c_void_p(d_out),
                                   C.c_void_p(stream) if stream else None))
+
+    def empty(self, shape, dtype) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a: np.ndarray) -> DeviceArray:
+        return DeviceArray.from_numpy(self, a)
+
+    def copy_dev(self, dst: DeviceArray, src: DeviceArray):
+        check(lib().de_copy_dev(self.h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr),
+                                min(dst.nbytes, src.nbytes)))
+
+    def fill_dev(self, dst: DeviceArray, byte: int):
+        check(lib().de_fill_dev(self.h, C.c_void_p(dst.ptr), int(byte), dst.nbytes))
+
+    # --- batches (device pointers) ---
+    def dense_batch_dev(self, pairs, cfgs, stream: int = 0):
+        P, Cs, n = _pairs_cfgs(pairs, cfgs)
+        check(lib().de_dense_batch_dev(self.h, P, Cs, n, C.c_void_p(stream) if stream else None))
+
+    def plan(self, pairs, cfgs) -> Plan:
+        return Plan(self, pairs, cfgs)
+
+    def autotune(self, W: int, H: int, cfg: Cfg, reps: int = 5):
+        rv, cv = C.c_int(), C.c_int()
+        c = cfg.to_c()
+        check(lib().de_autotune(self.h, W, H, C.byref(c), reps, C.byref(rv), C.byref(cv)))
+        return rv.value, cv.value
 
     def set_variants(self, row: int = -1, col: int = -1):
         check(lib().de_ctx_set_variants(self.h, row, col))

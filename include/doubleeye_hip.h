@@ -68,6 +68,17 @@ int de_ctx_set_variants(de_ctx* ctx, int row_variant, int col_variant);
 /* Message of the last failure on this thread ("" if none). */
 const char* de_last_error(void);
 
+/* Device memory through the library's own HIP runtime (callers that have no
+ * HIP of their own, e.g. ctypes users, the bench and the tests). Copies are on
+ * the context's stream and synchronous. */
+int de_alloc(de_ctx* ctx, size_t bytes, void** d_ptr);
+int de_free(de_ctx* ctx, void* d_ptr);
+int de_upload(de_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int de_download(de_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+/* Plain device copy / byte fill on the context's stream (asynchronous). */
+int de_copy_dev(de_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+int de_fill_dev(de_ctx* ctx, void* d_dst, int byte, size_t bytes);
+
 /* Census transform of one image, bit b in dy-major/dx-minor order with the
  * centre skipped, set iff neighbour < centre, 0 where the window is cut.
  * Replaces census_transform (core/include/doubleeye/preproc.hpp:92,
@@ -88,6 +99,49 @@ int de_dense_run(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
 int de_dense_run_dev(de_ctx* ctx, const uint8_t* d_L, const uint8_t* d_R, int W,
                      int H, const de_cfg* cfg, float* d_disp, float* d_margin,
                      float* d_cand6, void* stream);
+
+/* ---- Batches: many pairs of mixed size and disparity range, one launch per
+ * stage (plus one column-pass launch per disparity-group count). No
+ * counterpart in the reference, which runs one fixed-geometry pair at a time
+ * (core/tools/de_dense_cuda.cu:670-770). */
+
+/* One pair of a batch. Device pointers; conf and q4 (nullable) are the DEDD
+ * output planes of core/tools/de_dense_cuda.cu:731-745 (u8 confidence, Q4
+ * disparity with -32768 = no answer). */
+typedef struct de_pair {
+  const uint8_t* L;
+  const uint8_t* R;
+  int W, H;
+  float* disp;
+  float* margin;
+  float* cand6;
+  uint8_t* conf;
+  int16_t* q4;
+} de_pair;
+
+/* Run n pairs with per-pair configurations (D, dmin, iters and the gates may
+ * differ; the census window, sigma_s, sigma_r, ad and ad_trunc must be equal).
+ * Results are identical to n separate de_dense_run_dev calls. */
+int de_dense_batch_dev(de_ctx* ctx, const de_pair* pairs, const de_cfg* cfgs, int n,
+                       void* stream);
+
+/* A captured batch: its own resident scratch, descriptors uploaded once and
+ * the stage launches recorded in a hipGraph. Launching replays the graph on
+ * `stream`; the caller refreshes the pairs' input buffers before each launch
+ * (e.g. hipMemcpyAsync on the same stream). Plans of one context may be in
+ * flight concurrently on different streams. */
+typedef struct de_plan de_plan;
+int de_plan_create(de_ctx* ctx, const de_pair* pairs, const de_cfg* cfgs, int n,
+                   de_plan** out);
+int de_plan_launch(de_plan* plan, void* stream);
+void de_plan_destroy(de_plan* plan);
+
+/* Autotuner: times every prefetch-depth variant of the row and column passes
+ * on a synthetic pair of this shape (reps launches each), remembers the
+ * fastest for the shape bucket (W, H, disparity-group count) and returns it.
+ * Later launches of that bucket (plans included) use it. */
+int de_autotune(de_ctx* ctx, int W, int H, const de_cfg* cfg, int reps, int* row_variant,
+                int* col_variant);
 
 /* Intermediate for parity tests: the aggregated int16 Q14 volume for
  * disparities [k0, k1), layout [k][y][x] (the plane rf_filter_i16 leaves,

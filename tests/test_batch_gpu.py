@@ -1,0 +1,114 @@
+"""Batched dense MASDA (one launch per stage over uneven pairs), captured
+hipGraph plans and the autotuner: every result must equal the per-pair plain
+launch bit for bit (same kernels, deterministic reductions), and the plain
+launch is held to the oracle by test_dense_gpu.py."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.dtype == np.float32:
+        return bool(((a.view(np.uint32) == b.view(np.uint32)) | (np.isnan(a) & np.isnan(b))).all())
+    return bool((a == b).all())
+
+
+def make_pairs(ctx, shapes, seed=0):
+    from doubleeye_amd import synth
+    from doubleeye_amd.dense import Pair
+    out = []
+    for i, (W, H, D) in enumerate(shapes):
+        L, R, _ = synth.pair_for(W, H, D, seed + i, kind="fronto" if i % 2 else "slanted")
+        t = dict(L=ctx.to_device(L), R=ctx.to_device(R), disp=ctx.empty((H, W), np.float32),
+                 margin=ctx.empty((H, W), np.float32), cand=ctx.empty((6, H, W), np.float32),
+                 conf=ctx.empty((H, W), np.uint8), q4=ctx.empty((H, W), np.int16),
+                 Lh=L, Rh=R, W=W, H=H, D=D)
+        t["pair"] = Pair(t["L"].ptr, t["R"].ptr, W, H, t["disp"].ptr, t["margin"].ptr,
+                         t["cand"].ptr, t["conf"].ptr, t["q4"].ptr)
+        out.append(t)
+    return out
+
+
+SHAPES = [(320, 240, 64), (424, 240, 96), (640, 360, 128), (200, 120, 192),
+          (848, 480, 128), (300, 100, 64), (1280, 720, 192), (424, 240, 64)]
+
+
+def cfg_for(D, i):
+    from doubleeye_amd.dense import Cfg
+    return Cfg(dmax=D, iters=(8 if i % 3 else 0), sigma_s=20.0, sigma_r=0.0392,
+               min_margin=(0.01 if i == 5 else 0.0))
+
+
+def test_batch_equals_per_pair(ctx):
+    items = make_pairs(ctx, SHAPES)
+    cfgs = [cfg_for(t["D"], i) for i, t in enumerate(items)]
+    note(f"[batch] {len(items)} pairs {SHAPES}")
+    ctx.dense_batch_dev([t["pair"] for t in items], cfgs)
+    ctx.sync()
+    got = [(t["disp"].numpy(), t["margin"].numpy(), t["cand"].numpy(), t["conf"].numpy(),
+            t["q4"].numpy()) for t in items]
+    for i, t in enumerate(items):
+        note(f"[batch] per-pair check {i} {t['W']}x{t['H']}x{t['D']}")
+        d, m, c = ctx.run_dense(t["Lh"], t["Rh"], cfgs[i], want_cand=True)
+        assert same(got[i][0], d), f"pair {i}: disparity differs from the plain launch"
+        assert same(got[i][1], m), f"pair {i}: margin differs"
+        assert same(got[i][2], c), f"pair {i}: candidate planes differ"
+        # DEDD planes: Q4 = lround(16 d) where d > 0 and finite, else -32768
+        q = np.where(np.isfinite(d) & (d > 0), np.floor(np.nan_to_num(d) * np.float32(16) + 0.5),
+                     -32768)
+        assert same(got[i][4], q.astype(np.int16)), f"pair {i}: Q4 plane differs"
+        assert (got[i][3][q == -32768] == 0).all()
+
+
+def test_plan_graph_replays(ctx):
+    shapes = [(424, 240, 64), (320, 240, 96), (640, 360, 128)]
+    items = make_pairs(ctx, shapes, seed=40)
+    cfgs = [cfg_for(t["D"], i) for i, t in enumerate(items)]
+    note("[plan] create")
+    plan = ctx.plan([t["pair"] for t in items], cfgs)
+    try:
+        for rep in range(3):
+            # new inputs into the plan's fixed buffers, then replay
+            from doubleeye_amd import synth
+            for i, t in enumerate(items):
+                L, R, _ = synth.pair_for(t["W"], t["H"], t["D"], 100 + rep + i)
+                t["L"].upload(L); t["R"].upload(R)
+                t["Lh"], t["Rh"] = L, R
+            note(f"[plan] replay {rep}")
+            plan.launch(ctx.stream)
+            ctx.sync()
+            for i, t in enumerate(items):
+                d, m, _ = ctx.run_dense(t["Lh"], t["Rh"], cfgs[i])
+                assert same(t["disp"].numpy(), d), f"replay {rep} pair {i}"
+                assert same(t["margin"].numpy(), m)
+    finally:
+        plan.close()
+
+
+def test_variants_do_not_change_results(ctx):
+    from doubleeye_amd import synth
+    from doubleeye_amd.dense import Cfg
+    L, R, _ = synth.slanted_pair(424, 240, seed=7, dhi=80.0)
+    cfg = Cfg(dmax=96, iters=4, sigma_s=20.0, sigma_r=0.0392)
+    ref = ctx.run_dense(L, R, cfg, want_cand=True)
+    try:
+        for rv in range(4):
+            for cv in range(3):
+                ctx.set_variants(rv, cv)
+                got = ctx.run_dense(L, R, cfg, want_cand=True)
+                assert all(same(a, b) for a, b in zip(got, ref)), (rv, cv)
+    finally:
+        ctx.set_variants(-1, -1)
+    note("[autotune] 424x240 D=96")
+    rv, cv = ctx.autotune(424, 240, cfg, reps=3)
+    assert 0 <= rv < 4 and 0 <= cv < 3
+    assert all(same(a, b) for a, b in zip(ctx.run_dense(L, R, cfg, want_cand=True), ref))
