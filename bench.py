@@ -50,6 +50,90 @@ def algorithmic_bytes(w: int, h: int, d: int) -> dict:
     }
 
 
+C4_SIZES = [(320, 240), (424, 240), (640, 360), (848, 480), (1280, 720)]
+C4_DISP = [64, 96, 128, 192]
+
+
+def bench_c4(a):
+    """C4: 64 uneven pairs (sizes and disparity ranges drawn uniformly from the
+    sets above, seed 4), 9x7 census, 8 BP iterations. The batch runs as one
+    captured plan (one launch per stage, per-bucket hipGraph); the baseline is
+    the same 64 pairs as plain one-pair-per-launch calls. Both must agree bit
+    for bit."""
+    import numpy as np
+
+    from doubleeye_amd import synth
+    from doubleeye_amd.dense import Cfg, Context, Pair
+
+    rng = np.random.default_rng(4)
+    draws = [(C4_SIZES[int(rng.integers(len(C4_SIZES)))], C4_DISP[int(rng.integers(len(C4_DISP)))])
+             for _ in range(64)]
+    ctx = Context(0)
+    pool = {}
+    items = []
+    for i, ((w, h), d) in enumerate(draws):
+        if (w, h, d) not in pool:
+            pool[(w, h, d)] = synth.pair_for(w, h, d, seed=len(pool) + 11, kind="fronto")[:2]
+        L, R = pool[(w, h, d)]
+        t = dict(L=ctx.to_device(L), R=ctx.to_device(R), disp=ctx.empty((h, w), np.float32),
+                 margin=ctx.empty((h, w), np.float32), W=w, H=h, D=d)
+        t["pair"] = Pair(t["L"].ptr, t["R"].ptr, w, h, t["disp"].ptr, t["margin"].ptr)
+        t["cfg"] = Cfg(dmax=d, iters=8, sigma_s=20.0, sigma_r=0.0392, census_hw=4, census_hh=3)
+        items.append(t)
+    pxd = sum(t["W"] * t["H"] * t["D"] for t in items)
+
+    def plain():
+        for t in items:
+            ctx.run_dense_dev(t["L"].ptr, t["R"].ptr, t["W"], t["H"], t["cfg"], t["disp"].ptr,
+                              t["margin"].ptr)
+
+    plain()
+    ctx.sync()
+    ref = [t["disp"].numpy() for t in items]
+    for t in items:
+        ctx.fill_dev(t["disp"], 0)
+    plan = ctx.plan([t["pair"] for t in items], [t["cfg"] for t in items])
+    plan.launch()
+    ctx.sync()
+    same = all(np.array_equal(t["disp"].numpy().view(np.uint32), r.view(np.uint32))
+               for t, r in zip(items, ref))
+    reps = max(2, a.steps // 20)
+    for _ in range(2):
+        plan.launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.launch()
+    ctx.sync()
+    ms_batch = (time.perf_counter() - t0) * 1e3 / reps
+    plain()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain()
+    ctx.sync()
+    ms_plain = (time.perf_counter() - t0) * 1e3 / reps
+    plan.close()
+    out = {
+        "metric": "batched dense Mpx*disp/s, 64 uneven pairs (C4)",
+        "value": round(pxd / (ms_batch * 1e-3) / 1e6, 1), "unit": "Mpx*disp/s",
+        "n_gpus": 1, "steps": reps, "warmup": 2, "ms_per_step": round(ms_batch, 3),
+        "frames_per_s": round(64e3 / ms_batch, 1),
+        "plain_ms_per_batch": round(ms_plain, 3),
+        "plain_value": round(pxd / (ms_plain * 1e-3) / 1e6, 1),
+        "batched_over_plain": round(ms_plain / ms_batch, 3),
+        "identical_to_plain": same,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i16 volume / f32 messages",
+        "data": "synthetic (fronto-plane pairs, doubleeye_amd.synth, draw seed 4)",
+        "config": {"workload": "64 pairs, sizes {320x240,424x240,640x360,848x480,1280x720}, "
+                               "D {64,96,128,192}, census 9x7, 8 BP iterations",
+                   "px_disp_per_batch": pxd},
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(L, R, budget_s: float = 12.0):
     """Reference CPU tool at --threads 1 on the same pair, repeated to ~budget_s."""
     ref = os.path.join(ROOT, "oracle", "_ref", "de_dense")
@@ -92,6 +176,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4"],
+                    help="c2: one 848x480x128 pair (default, the headline); c4: 64-pair batch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one plain device-to-device copy of MIB MiB (the "
                          "known-size kernel that scales rocprofv3 FETCH/WRITE_SIZE)")
@@ -101,6 +187,9 @@ def main():
               "(the dense frame shards by disparity range or by stream; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "c4":
+        bench_c4(a)
+        return
 
     import numpy as np
 
