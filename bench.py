@@ -184,7 +184,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the dense frame shards by disparity range or by stream; see DESIGN.md)",
+              "(a dense frame would shard by stream, or by disparity range with a top-2 merge)",
               file=sys.stderr)
         sys.exit(2)
     if a.workload == "c4":

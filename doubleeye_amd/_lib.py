@@ -44,6 +44,41 @@ class DePairC(C.Structure):
                 ("conf", C.c_void_p), ("q4", C.c_void_p)]
 
 
+class DeMatchCfgC(C.Structure):
+    """de_match_cfg (include/doubleeye_hip.h)."""
+    _fields_ = [("max_dy", C.c_float), ("min_disparity", C.c_float),
+                ("max_disparity", C.c_float), ("max_candidates", C.c_int),
+                ("sigma_y", C.c_float), ("w_y", C.c_float), ("w_prior", C.c_float),
+                ("w_smooth", C.c_float), ("iterations", C.c_int), ("damping", C.c_float),
+                ("lambda_", C.c_float), ("gamma", C.c_float), ("converge_eps", C.c_float)]
+
+
+class DePriorC(C.Structure):
+    """de_disparity_prior (include/doubleeye_hip.h)."""
+    _fields_ = [("cell", C.c_int), ("cols", C.c_int), ("rows", C.c_int),
+                ("disparity", C.c_void_p), ("sigma", C.c_void_p), ("valid", C.c_void_p)]
+
+
+class DeSmoothC(C.Structure):
+    """de_smooth_prior (include/doubleeye_hip.h)."""
+    _fields_ = [("n", C.c_int), ("disparity", C.c_void_p), ("sigma", C.c_void_p),
+                ("valid", C.c_void_p)]
+
+
+class DeMatchStatsC(C.Structure):
+    """de_match_stats (include/doubleeye_hip.h)."""
+    _fields_ = [("left", C.c_int), ("right", C.c_int), ("candidates", C.c_int),
+                ("iterations_run", C.c_int), ("converged", C.c_int),
+                ("final_max_delta", C.c_float), ("oscillating", C.c_int)]
+
+
+class DeSparseFrameC(C.Structure):
+    """de_sparse_frame (include/doubleeye_hip.h): device pointers of one frame."""
+    _fields_ = [("left", C.c_void_p), ("desc_left", C.c_void_p), ("n_left", C.c_int),
+                ("right", C.c_void_p), ("desc_right", C.c_void_p), ("n_right", C.c_int),
+                ("out", C.c_void_p), ("n_out", C.c_void_p), ("stats", C.c_void_p)]
+
+
 # (name, restype, argtypes) for every exported entry point
 P = C.c_void_p
 I = C.c_int
@@ -75,6 +110,15 @@ SIGNATURES = {
     "de_plan_launch": (I, [P, P]),
     "de_plan_destroy": (None, [P]),
     "de_autotune": (I, [P, I, I, C.POINTER(DeCfgC), I, C.POINTER(I), C.POINTER(I)]),
+    "de_match_default_cfg": (None, [C.POINTER(DeMatchCfgC)]),
+    "de_match_masda": (I, [P, P, P, I, P, P, I, C.POINTER(DeMatchCfgC), P, P, P,
+                           C.POINTER(I), P]),
+    "de_match_mutual_nn": (I, [P, P, P, I, P, P, I, C.POINTER(DeMatchCfgC), P, P, F, P,
+                               C.POINTER(I), P]),
+    "de_match_candidates": (I, [P, P, P, I, P, P, I, C.POINTER(DeMatchCfgC), P, P, P, P, P,
+                                C.POINTER(I)]),
+    "de_match_masda_batch_dev": (I, [P, C.POINTER(DeSparseFrameC), C.POINTER(DeMatchCfgC),
+                                     I, P]),
 }
 
 _lib = None

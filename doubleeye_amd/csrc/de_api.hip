@@ -36,11 +36,6 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(DE_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-#define DE_HIP(call)                                        \
-  do {                                                      \
-    hipError_t e_ = (call);                                 \
-    if (e_ != hipSuccess) return hip_fail(e_, #call);       \
-  } while (0)
 
 // ---------------------------------------------------------------------------
 // device buffers
@@ -382,10 +377,6 @@ Ctx::~Ctx() {
 }  // namespace de
 
 using namespace de;
-
-struct de_ctx {
-  Ctx c;
-};
 
 extern "C" {
 

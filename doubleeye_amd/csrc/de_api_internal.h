@@ -17,6 +17,13 @@ namespace de {
 
 int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
+
+// Return the library status of a failing HIP call.
+#define DE_HIP(call)                                        \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);       \
+  } while (0)
 int validate(const de_cfg& c, int W, int H);
 
 struct DevBuf {
@@ -50,6 +57,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   std::mutex mu;
   DevBuf scratch, rf_buf, sc_buf, io_in, io_out;
+  DevBuf sp_scratch, sp_io;   // sparse matcher: per-frame scratch, staged inputs/outputs
   Slot slots[kSlots];
   int next_slot = 0;
   bool store_vol = false;
@@ -109,5 +117,10 @@ hipError_t launch_census_plane(const uint8_t* img, int W, int H, int hw, int hh,
                                uint64_t* out, hipStream_t s);
 
 }  // namespace de
+
+// The opaque context handle of the C ABI.
+struct de_ctx {
+  de::Ctx c;
+};
 
 #endif

@@ -18,7 +18,8 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["random_dots", "fronto_pair", "slanted_pair", "pair_for"]
+__all__ = ["random_dots", "fronto_pair", "slanted_pair", "pair_for", "census_at",
+           "sparse_problem"]
 
 
 def _gauss_kernel(sigma: float) -> np.ndarray:
@@ -165,3 +166,64 @@ def pair_for(w: int, h: int, dmax: int, seed: int, kind: str = "fronto"):
         ds = tuple(max(1, int(round(v * s))) for v in (8, 20, 35))
         return fronto_pair(w, h, seed, ds)
     return slanted_pair(w, h, seed, 4.0, 0.86 * dmax)
+
+
+def census_at(img: np.ndarray, xs: np.ndarray, ys: np.ndarray, hw: int = 3,
+              hh: int = 3) -> np.ndarray:
+    """Census descriptors at integer pixel positions (numpy): bit b is set when
+    the b-th neighbour, dy-major then dx-minor with the centre skipped, is darker
+    than the centre (the definition of core/src/preproc.cpp:110-162). Points
+    whose window crosses the border get 0."""
+    img = np.asarray(img, np.int32)
+    H, W = img.shape
+    xs = np.asarray(xs, np.int64)
+    ys = np.asarray(ys, np.int64)
+    inside = (xs >= hw) & (xs < W - hw) & (ys >= hh) & (ys < H - hh)
+    xc = np.clip(xs, hw, W - hw - 1)
+    yc = np.clip(ys, hh, H - hh - 1)
+    c = img[yc, xc]
+    d = np.zeros(len(xs), np.uint64)
+    b = 0
+    for dy in range(-hh, hh + 1):
+        for dx in range(-hw, hw + 1):
+            if dx == 0 and dy == 0:
+                continue
+            bit = (img[yc + dy, xc + dx] < c).astype(np.uint64) << np.uint64(b)
+            d |= bit
+            b += 1
+    d[~inside] = 0
+    return d
+
+
+def sparse_problem(n: int = 2000, w: int = 848, h: int = 480, seed: int = 3,
+                   keep: float = 0.9, outliers: float = 0.1, jitter: float = 0.3):
+    """A keypoint matching problem on the fronto random-dot pair: n left
+    keypoints drawn uniformly (sub-pixel positions) with 7x7 census
+    descriptors; right keypoints are their true correspondences (disparity from
+    the ground truth) with N(0, jitter) position noise, a `keep` fraction of
+    them, plus `outliers` * n unrelated right keypoints, in shuffled order.
+
+    Returns (left_xy, desc_left, right_xy, desc_right, truth) where truth[k] is
+    the left index of right keypoint k or -1."""
+    L, R, gt = fronto_pair(w, h, seed)
+    rng = np.random.default_rng(seed + 7919)
+    border = 8
+    lx = rng.uniform(border, w - border, n).astype(np.float32)
+    ly = rng.uniform(border, h - border, n).astype(np.float32)
+    dl = census_at(L, np.rint(lx), np.rint(ly))
+    d = gt[np.rint(ly).astype(int), np.rint(lx).astype(int)].astype(np.float32)
+    sel = np.nonzero(rng.random(n) < keep)[0]
+    rx = lx[sel] - d[sel] + rng.normal(0, jitter, len(sel)).astype(np.float32)
+    ry = ly[sel] + rng.normal(0, jitter, len(sel)).astype(np.float32)
+    no = int(round(outliers * n))
+    ox = rng.uniform(border, w - border, no).astype(np.float32)
+    oy = rng.uniform(border, h - border, no).astype(np.float32)
+    rx = np.concatenate([rx, ox]).astype(np.float32)
+    ry = np.concatenate([ry, oy]).astype(np.float32)
+    truth = np.concatenate([sel, -np.ones(no, np.int64)])
+    ok = (rx >= border) & (rx < w - border) & (ry >= border) & (ry < h - border)
+    rx, ry, truth = rx[ok], ry[ok], truth[ok]
+    perm = rng.permutation(len(rx))
+    rx, ry, truth = rx[perm], ry[perm], truth[perm]
+    dr = census_at(R, np.rint(rx), np.rint(ry))
+    return (np.stack([lx, ly], 1), dl, np.stack([rx, ry], 1), dr, truth)

@@ -155,6 +155,111 @@ int de_dense_volume(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int 
 int de_dense_candidates(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W,
                         int H, const de_cfg* cfg, uint32_t* out);
 
+/* ==== Sparse keypoint MASDA (core/src/match.cpp) ==========================
+ * Left keypoints are measurements, right keypoints objects; candidate edges
+ * join a left keypoint to right keypoints in its y band and disparity window,
+ * scored by Census agreement and the y residual (plus optional soft priors).
+ * Results equal the reference's whenever no exact tie decides an order; at
+ * exact ties every order is broken by position (right keypoints by (y, index),
+ * kept candidates by (score, band position), greedy completion by (belief,
+ * score, edge index)), where the reference follows libstdc++'s unstable sorts. */
+
+/* doubleeye::Keypoint (core/include/doubleeye/preproc.hpp:125-130) */
+typedef struct de_keypoint {
+  float x, y, response, local_std;
+} de_keypoint;
+
+/* doubleeye::Match (core/include/doubleeye/match.hpp:46-65) */
+typedef struct de_match {
+  int left, right;
+  float score, belief, disparity, dy, margin;
+} de_match;
+
+/* doubleeye::MatchConfig (match.hpp:136-176); its prior and smooth pointers
+ * are the separate arguments below. */
+typedef struct de_match_cfg {
+  float max_dy, min_disparity, max_disparity;
+  int max_candidates;
+  float sigma_y, w_y, w_prior, w_smooth;
+  int iterations;
+  float damping, lambda, gamma, converge_eps;
+} de_match_cfg;
+
+/* doubleeye::DisparityPrior (match.hpp:81-92): cols*rows cells of `cell` px. */
+typedef struct de_disparity_prior {
+  int cell, cols, rows;
+  const float* disparity;
+  const float* sigma;
+  const uint8_t* valid;
+} de_disparity_prior;
+
+/* doubleeye::SmoothPrior (match.hpp:110-114): per left keypoint. */
+typedef struct de_smooth_prior {
+  int n;
+  const float* disparity;
+  const float* sigma;
+  const uint8_t* valid;
+} de_smooth_prior;
+
+/* doubleeye::MatchStats (match.hpp:178-186) */
+typedef struct de_match_stats {
+  int left, right, candidates, iterations_run, converged;
+  float final_max_delta;
+  int oscillating;
+} de_match_stats;
+
+/* Right keypoints per frame (the y sort runs in one workgroup's LDS). */
+#define DE_SPARSE_MAX_RIGHT 8192
+
+/* MatchConfig's defaults (match.hpp:136-176). */
+void de_match_default_cfg(de_match_cfg* cfg);
+
+/* match_masda (match.hpp:221-226, match.cpp:334-417). Host pointers; prior and
+ * smooth are nullable. `out` holds min(n_left, n_right) matches; *n_out gets
+ * the count; stats is nullable. */
+int de_match_masda(de_ctx* ctx, const de_keypoint* left, const uint64_t* desc_left,
+                   int n_left, const de_keypoint* right, const uint64_t* desc_right,
+                   int n_right, const de_match_cfg* cfg, const de_disparity_prior* prior,
+                   const de_smooth_prior* smooth, de_match* out, int* n_out,
+                   de_match_stats* stats);
+
+/* match_mutual_nn (match.hpp:228-235, match.cpp:419-476): mutual nearest
+ * neighbour with a ratio test over the same candidate graph. */
+int de_match_mutual_nn(de_ctx* ctx, const de_keypoint* left, const uint64_t* desc_left,
+                       int n_left, const de_keypoint* right, const uint64_t* desc_right,
+                       int n_right, const de_match_cfg* cfg,
+                       const de_disparity_prior* prior, const de_smooth_prior* smooth,
+                       float ratio, de_match* out, int* n_out, de_match_stats* stats);
+
+/* The candidate graph build_candidates (match.cpp:46-102) produces, in the
+ * solver's edge order: edge e joins ei[e] to ej[e] with score es[e]. Arrays
+ * hold n_left * max(1, max_candidates) entries. For parity tests. */
+int de_match_candidates(de_ctx* ctx, const de_keypoint* left, const uint64_t* desc_left,
+                        int n_left, const de_keypoint* right, const uint64_t* desc_right,
+                        int n_right, const de_match_cfg* cfg,
+                        const de_disparity_prior* prior, const de_smooth_prior* smooth,
+                        int* ei, int* ej, float* es, int* n_edges);
+
+/* One sparse frame of a device batch. Device pointers; out holds
+ * min(n_left, n_right) matches; n_out and stats are single device records. */
+typedef struct de_sparse_frame {
+  const de_keypoint* left;
+  const uint64_t* desc_left;
+  int n_left;
+  const de_keypoint* right;
+  const uint64_t* desc_right;
+  int n_right;
+  de_match* out;
+  int* n_out;
+  de_match_stats* stats;
+} de_sparse_frame;
+
+/* match_masda over n frames with per-frame configurations, three launches in
+ * all (y sort, candidates, solve: one workgroup per frame). No priors. Results
+ * equal n separate de_match_masda calls. No counterpart in the reference. */
+int de_match_masda_batch_dev(de_ctx* ctx, const de_sparse_frame* frames,
+                             const de_match_cfg* cfgs, int n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
