@@ -169,6 +169,85 @@ def cpu_baseline(L, R, budget_s: float = 12.0):
                       f"mean {ms:.1f} ms/frame"}
 
 
+def bench_sparse(a):
+    """Sparse keypoint MASDA (match_masda): 2000 left keypoints on the 848x480
+    random-dot pair, 8 candidates, 10 iterations, inputs resident in HBM, one
+    frame per launch sequence and a batch of 16 frames. The CPU baseline is the
+    reference's own match_masda (oracle/_ref/libref_sparse.so, built from
+    core/src/match.cpp), single thread, on the same frame."""
+    import numpy as np
+
+    from doubleeye_amd import sparse, synth
+    from doubleeye_amd.dense import Context
+
+    ctx = Context(0)
+    cfg = sparse.MatchConfig(max_candidates=8, iterations=10)
+    probs = [synth.sparse_problem(2000, 848, 480, seed) for seed in range(3, 3 + 16)]
+
+    def frame(p):
+        lxy, dl, rxy, dr, _ = p
+        L, R = sparse.keypoints(lxy), sparse.keypoints(rxy)
+        return sparse.SparseFrame(ctx.to_device(L), ctx.to_device(dl), ctx.to_device(R),
+                                  ctx.to_device(dr), ctx.empty((min(len(L), len(R)),), sparse.MATCH),
+                                  ctx.empty((1,), np.int32), ctx.empty((1,), sparse.STATS))
+
+    frames = [frame(p) for p in probs]
+    one = frames[:1]
+    want = sparse.match_masda(*probs[0][:4], cfg, ctx=ctx)
+    sparse.match_masda_batch_dev(ctx, one, [cfg])
+    ctx.sync()
+    n0 = int(one[0].n_out.numpy()[0])
+    same = bool(n0 == len(want) and np.array_equal(one[0].out.numpy()[:n0].view(np.uint8),
+                                                   want.view(np.uint8)))
+    edges = int(one[0].stats.numpy()[0]["candidates"])
+    reps = max(20, a.steps)
+    for _ in range(5):
+        sparse.match_masda_batch_dev(ctx, one, [cfg])
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sparse.match_masda_batch_dev(ctx, one, [cfg])
+    ctx.sync()
+    ms_one = (time.perf_counter() - t0) * 1e3 / reps
+    sparse.match_masda_batch_dev(ctx, frames, [cfg] * len(frames))
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sparse.match_masda_batch_dev(ctx, frames, [cfg] * len(frames))
+    ctx.sync()
+    ms_batch = (time.perf_counter() - t0) * 1e3 / reps
+    out = {
+        "metric": "sparse match_masda frames/s, 2000 keypoints (848x480)",
+        "value": round(1e3 / ms_one, 1), "unit": "frames/s",
+        "n_gpus": 1, "steps": reps, "warmup": 5, "ms_per_step": round(ms_one, 4),
+        "edges": edges, "edges_per_s": round(edges / (ms_one * 1e-3), 1),
+        "batch16_ms": round(ms_batch, 4), "batch16_frames_per_s": round(16e3 / ms_batch, 1),
+        "identical_to_single_call": same,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 messages / u64 census", "data": "synthetic (doubleeye_amd.synth.sparse_problem)",
+        "config": {"workload": "match_masda, 2000 left kp, max_candidates 8, 10 iterations"},
+    }
+    if not a.no_cpu_baseline:
+        try:
+            sys.path.insert(0, ROOT)
+            from oracle import sparse_binding as sb
+            c = sb.match_cfg(max_candidates=8, iterations=10)
+            lxy, dl, rxy, dr, _ = probs[0]
+            sb.ref_match_masda(lxy, dl, rxy, dr, c)
+            n, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < min(a.cpu_budget, 5.0):
+                sb.ref_match_masda(lxy, dl, rxy, dr, c)
+                n += 1
+            ms = (time.perf_counter() - t0) * 1e3 / n
+            out["cpu_baseline"] = {"value": round(1e3 / ms, 1), "unit": "frames/s", "cores": 1,
+                                   "kind": "reference", "ms_per_frame": round(ms, 3),
+                                   "sample": f"{n} runs of the same frame"}
+        except Exception as e:   # reference harness absent on this host
+            out["cpu_baseline"] = {"error": str(e)}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,8 +255,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4"],
-                    help="c2: one 848x480x128 pair (default, the headline); c4: 64-pair batch")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "sparse"],
+                    help="c2: one 848x480x128 pair (default, the headline); c4: 64-pair "
+                         "batch; sparse: keypoint match_masda")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one plain device-to-device copy of MIB MiB (the "
                          "known-size kernel that scales rocprofv3 FETCH/WRITE_SIZE)")
@@ -189,6 +269,9 @@ def main():
         sys.exit(2)
     if a.workload == "c4":
         bench_c4(a)
+        return
+    if a.workload == "sparse":
+        bench_sparse(a)
         return
 
     import numpy as np

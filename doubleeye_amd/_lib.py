@@ -79,6 +79,19 @@ class DeSparseFrameC(C.Structure):
                 ("out", C.c_void_p), ("n_out", C.c_void_p), ("stats", C.c_void_p)]
 
 
+class DeKnnCfgC(C.Structure):
+    """de_knn_cfg (include/doubleeye_hip.h)."""
+    _fields_ = [("census_hw", C.c_int), ("census_hh", C.c_int), ("dmax", C.c_int),
+                ("n_cand", C.c_int), ("k_nn", C.c_int), ("iters", C.c_int),
+                ("w_data", C.c_float), ("lambda_", C.c_float), ("trunc", C.c_int)]
+
+
+class DeKnnFrameC(C.Structure):
+    """de_knn_frame (include/doubleeye_hip.h): device pointers of one frame."""
+    _fields_ = [("L", C.c_void_p), ("R", C.c_void_p), ("W", C.c_int), ("H", C.c_int),
+                ("kp", C.c_void_p), ("n", C.c_int), ("out", C.c_void_p)]
+
+
 # (name, restype, argtypes) for every exported entry point
 P = C.c_void_p
 I = C.c_int
@@ -119,6 +132,12 @@ SIGNATURES = {
                                 C.POINTER(I)]),
     "de_match_masda_batch_dev": (I, [P, C.POINTER(DeSparseFrameC), C.POINTER(DeMatchCfgC),
                                      I, P]),
+    "de_knn_default_cfg": (None, [C.POINTER(DeKnnCfgC)]),
+    "de_knn_bp": (I, [P, P, P, I, I, P, I, C.POINTER(DeKnnCfgC), P]),
+    "de_knn_candidates": (I, [P, P, P, I, I, P, I, C.POINTER(DeKnnCfgC), P, P, P]),
+    "de_knn_graph": (I, [P, P, P, I, I, P, I, C.POINTER(DeKnnCfgC), P, P, I, C.POINTER(I)]),
+    "de_knn_bp_graph": (I, [P, I, P, P, P, P, P, C.POINTER(DeKnnCfgC), P, P]),
+    "de_knn_bp_batch_dev": (I, [P, C.POINTER(DeKnnFrameC), C.POINTER(DeKnnCfgC), I, P]),
 }
 
 _lib = None

@@ -30,6 +30,7 @@ SOURCES = {
     "de_solve.hip": ["-ffp-contract=off"],
     "de_api.hip": ["-ffp-contract=off"],
     "de_sparse.hip": ["-ffp-contract=off"],
+    "de_knn.hip": ["-ffp-contract=off"],
 }
 
 

@@ -369,7 +369,8 @@ Ctx::~Ctx() {
   for (std::vector<EvSet>* v : {&ev_pending, &ev_pool})
     for (EvSet& ev : *v)
       for (int i = 0; i <= kNumStages; ++i) (void)hipEventDestroy(ev.e[i]);
-  for (DevBuf* b : {&scratch, &rf_buf, &sc_buf, &io_in, &io_out})
+  for (DevBuf* b : {&scratch, &rf_buf, &sc_buf, &io_in, &io_out, &sp_scratch, &sp_io,
+                    &kb_scratch, &kb_io})
     b->release();
   if (stream) (void)hipStreamDestroy(stream);
 }

@@ -42,6 +42,7 @@ namespace {
 constexpr int kSpThreads = 1024;      // solver / sort workgroup
 constexpr int kSpWaves = kSpThreads / kWave;
 constexpr int kCandWaves = 4;         // left keypoints per candidate workgroup
+constexpr int kChunk = 8;             // loads issued together in the solver loops
 constexpr float kNeg = -1e30f;
 
 enum SpMode { kModeMasda = 0, kModeMutualNN = 1, kModeCandidates = 2 };
@@ -346,6 +347,8 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
   __shared__ float shf[kSpWaves];
   __shared__ int shi[kSpWaves];
   __shared__ int sh_any, sh_n2;
+  __shared__ int tile_e[kSpThreads];
+  __shared__ float tile_b[kSpThreads], tile_s[kSpThreads];
   const SpDesc f = frames[blockIdx.x];
   const int tid = threadIdx.x, nl = f.nl, nr = f.nr, K = f.K;
 
@@ -473,23 +476,40 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
   float last = 1e30f;
   for (int it = 0; it < f.iterations; ++it) {
     float delta = 0.f;
-    // rho_ij = s_ij - max(lambda, max_{k != j} beta_ik): competition in a row
+    // rho_ij = s_ij - max(lambda, max_{k != j} beta_ik): competition in a row.
+    // Loads are issued kChunk at a time so their latencies overlap; the
+    // arithmetic runs in ascending k as the reference's loop does.
     for (int i = tid; i < nl; i += kSpThreads) {
       const int n = f.rc[i], base = i * K;
       float b1 = kNeg, b2 = kNeg;
       int arg = -1;
-      for (int k = 0; k < n; ++k) {
-        const float v = f.beta[base + k];
-        if (v > b1) { b2 = b1; b1 = v; arg = k; }
-        else if (v > b2) b2 = v;
+      for (int k0 = 0; k0 < n; k0 += kChunk) {
+        float v[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) v[u] = k0 + u < n ? f.beta[base + k0 + u] : kNeg;
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          if (k0 + u >= n) break;
+          if (v[u] > b1) { b2 = b1; b1 = v[u]; arg = k0 + u; }
+          else if (v[u] > b2) b2 = v[u];
+        }
       }
-      for (int k = 0; k < n; ++k) {
-        const float other = k == arg ? b2 : b1;
-        const float target = f.es[base + k] - fmax_ref(f.lambda, other);
-        const float old = f.rho[base + k];
-        const float nv = (1.f - f.damp) * target + f.damp * old;
-        delta = fmax_ref(delta, fabsf(nv - old));
-        f.rho[base + k] = nv;
+      for (int k0 = 0; k0 < n; k0 += kChunk) {
+        float sv[kChunk], ov[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          sv[u] = k0 + u < n ? f.es[base + k0 + u] : 0.f;
+          ov[u] = k0 + u < n ? f.rho[base + k0 + u] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          if (k0 + u >= n) break;
+          const float other = k0 + u == arg ? b2 : b1;
+          const float target = sv[u] - fmax_ref(f.lambda, other);
+          const float nv = (1.f - f.damp) * target + f.damp * ov[u];
+          delta = fmax_ref(delta, fabsf(nv - ov[u]));
+          f.rho[base + k0 + u] = nv;
+        }
       }
     }
     __syncthreads();
@@ -498,20 +518,39 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
       const int q0 = col_begin(j), q1 = col_end[j];
       float b1 = kNeg, b2 = kNeg;
       int arg = -1;
-      for (int q = q0; q < q1; ++q) {
-        const int e = f.col_edges[q];
-        const float v = f.rho[e];
-        if (v > b1) { b2 = b1; b1 = v; arg = e; }
-        else if (v > b2) b2 = v;
+      for (int c0 = q0; c0 < q1; c0 += kChunk) {
+        int ev[kChunk];
+        float v[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) ev[u] = c0 + u < q1 ? f.col_edges[c0 + u] : 0;
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) v[u] = c0 + u < q1 ? f.rho[ev[u]] : kNeg;
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          if (c0 + u >= q1) break;
+          if (v[u] > b1) { b2 = b1; b1 = v[u]; arg = ev[u]; }
+          else if (v[u] > b2) b2 = v[u];
+        }
       }
-      for (int q = q0; q < q1; ++q) {
-        const int e = f.col_edges[q];
-        const float other = e == arg ? b2 : b1;
-        const float target = f.es[e] - fmax_ref(f.gamma, other);
-        const float old = f.beta[e];
-        const float nv = (1.f - f.damp) * target + f.damp * old;
-        delta = fmax_ref(delta, fabsf(nv - old));
-        f.beta[e] = nv;
+      for (int c0 = q0; c0 < q1; c0 += kChunk) {
+        int ev[kChunk];
+        float sv[kChunk], ov[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) ev[u] = c0 + u < q1 ? f.col_edges[c0 + u] : 0;
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          sv[u] = c0 + u < q1 ? f.es[ev[u]] : 0.f;
+          ov[u] = c0 + u < q1 ? f.beta[ev[u]] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          if (c0 + u >= q1) break;
+          const float other = ev[u] == arg ? b2 : b1;
+          const float target = sv[u] - fmax_ref(f.gamma, other);
+          const float nv = (1.f - f.damp) * target + f.damp * ov[u];
+          delta = fmax_ref(delta, fabsf(nv - ov[u]));
+          f.beta[ev[u]] = nv;
+        }
       }
     }
     delta = block_max(delta, shf);
@@ -615,12 +654,32 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
     __syncthreads();
     if (!any) break;
   }
+  // Output order of the completion: rank among the accepted edges, counted
+  // against tiles of their keys staged in LDS.
   const int n2 = sh_n2;
-  for (int a = tid; a < n2; a += kSpThreads) {
-    const int e = f.acc2[a];
+  for (int a0 = 0; a0 < n2; a0 += kSpThreads) {
+    const int a = a0 + tid;
+    const int e = a < n2 ? f.acc2[a] : 0;
+    const float be = a < n2 ? f.bel[e] : 0.f, se = a < n2 ? f.es[e] : 0.f;
     int rank = 0;
-    for (int b2 = 0; b2 < n2; ++b2) rank += before_in_greedy(f, f.acc2[b2], e) ? 1 : 0;
-    put_match(f, n1 + rank, e, f.bel[e]);
+    for (int t0 = 0; t0 < n2; t0 += kSpThreads) {
+      __syncthreads();
+      if (t0 + tid < n2) {
+        const int x = f.acc2[t0 + tid];
+        tile_e[tid] = x;
+        tile_b[tid] = f.bel[x];
+        tile_s[tid] = f.es[x];
+      }
+      __syncthreads();
+      const int cnt = min(kSpThreads, n2 - t0);
+      if (a < n2) {
+        for (int b = 0; b < cnt; ++b) {
+          const float bb = tile_b[b], sb = tile_s[b];
+          rank += (bb != be) ? (bb > be) : (sb != se) ? (sb > se) : (tile_e[b] < e);
+        }
+      }
+    }
+    if (a < n2) put_match(f, n1 + rank, e, be);
   }
   if (tid == 0) {
     *f.n_out = n1 + n2;

@@ -260,6 +260,67 @@ typedef struct de_sparse_frame {
 int de_match_masda_batch_dev(de_ctx* ctx, const de_sparse_frame* frames,
                              const de_match_cfg* cfgs, int n, void* stream);
 
+/* ==== Keypoint-graph belief propagation ====================================
+ * No counterpart in the reference (its sparse matcher is bipartite). For
+ * keypoints of the left image: labels are the n_cand disparities along the
+ * same row with the smallest census Hamming distance (ties by smaller d), unary
+ * cost w_data * Hamming; the graph joins each keypoint to its k_nn nearest
+ * neighbours (symmetric union); min-sum loopy BP with the smoothness cost
+ * lambda * min(|d_i - d_j|, trunc) runs `iters` synchronous updates from zero
+ * messages; the output is the best and second label per keypoint. All float
+ * arithmetic follows one fixed order, so results are reproducible bit for bit. */
+typedef struct de_knn_cfg {
+  int census_hw, census_hh;  /* 4, 3: the 9x7 window */
+  int dmax;                  /* disparities 0..dmax, dmax <= 255 */
+  int n_cand;                /* labels per keypoint, 1..8 */
+  int k_nn;                  /* neighbours per keypoint, 0..8 */
+  int iters;
+  float w_data, lambda;
+  int trunc;                 /* px */
+} de_knn_cfg;
+
+typedef struct de_knn_out {
+  int d1, d2;                /* best and second disparity, -1 when absent */
+  float c1, c2;              /* their belief costs, +inf when absent */
+} de_knn_out;
+
+void de_knn_default_cfg(de_knn_cfg* cfg);
+
+/* The whole workload on one frame. Host pointers; kp is n (x, y) int pairs,
+ * each inside the census border. */
+int de_knn_bp(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+              const int32_t* kp, int n, const de_knn_cfg* cfg, de_knn_out* out);
+
+/* Intermediates for parity tests (host pointers): labels [n][8] (d = -1 and
+ * cost 0 past ncand[i]), and the graph as CSR (off [n+1], adj ascending). */
+int de_knn_candidates(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+                      const int32_t* kp, int n, const de_knn_cfg* cfg, int32_t* cand_d,
+                      float* cand_c, int32_t* ncand);
+int de_knn_graph(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+                 const int32_t* kp, int n, const de_knn_cfg* cfg, int32_t* off,
+                 int32_t* adj, int adj_cap, int* n_edges);
+
+/* BP alone on a given symmetric graph (ascending CSR lists, at most 64
+ * neighbours and 8 labels per node). beliefs is nullable, [n][8]. */
+int de_knn_bp_graph(de_ctx* ctx, int n, const int32_t* cand_d, const float* cand_c,
+                    const int32_t* ncand, const int32_t* off, const int32_t* adj,
+                    const de_knn_cfg* cfg, de_knn_out* out, float* beliefs);
+
+/* One frame of a device batch (device pointers). */
+typedef struct de_knn_frame {
+  const uint8_t* L;
+  const uint8_t* R;
+  int W, H;
+  const int32_t* kp;
+  int n;
+  de_knn_out* out;
+} de_knn_frame;
+
+/* The workload over n frames (equal iters), one launch per stage. Keypoints
+ * outside the census border get no labels (d1 = -1). */
+int de_knn_bp_batch_dev(de_ctx* ctx, const de_knn_frame* frames, const de_knn_cfg* cfgs,
+                        int n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
