@@ -248,6 +248,222 @@ def bench_sparse(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_c3(a):
+    """C3: keypoint-graph BP. 2000 keypoints drawn uniformly (seeded) on the
+    848x480 random-dot pair, 9x7 census, 8 labels from d in 0..127 by Hamming
+    distance, k = 8 nearest neighbours, 10 min-sum iterations with the
+    truncated-linear term (trunc 6 px), top-2 per keypoint. Inputs resident in
+    HBM; one frame per launch sequence, and a batch of 16 frames. The CPU
+    baseline is the oracle's restatement (the reference has no such solver),
+    single thread."""
+    import numpy as np
+
+    from doubleeye_amd import knn, synth
+    from doubleeye_amd.dense import Context
+
+    ctx = Context(0)
+    cfg = knn.KnnConfig()
+    W_, H_, N_ = 848, 480, 2000
+
+    def problem(seed):
+        L, R, _ = synth.fronto_pair(W_, H_, seed)
+        rng = np.random.default_rng(seed + 16)
+        kp = np.stack([rng.integers(4, W_ - 4, N_), rng.integers(3, H_ - 3, N_)], 1).astype(np.int32)
+        return L, R, kp
+
+    probs = [problem(s) for s in range(1, 17)]
+    frames = [knn.KnnFrame(ctx.to_device(L), ctx.to_device(R), ctx.to_device(kp),
+                           ctx.empty((N_,), knn.OUT)) for L, R, kp in probs]
+    one = frames[:1]
+    want = knn.knn_bp(*probs[0], cfg, ctx=ctx)
+    knn.knn_bp_batch_dev(ctx, one, [cfg])
+    ctx.sync()
+    got = one[0].out.numpy()
+    same = bool(np.array_equal(got.view(np.uint8), want.view(np.uint8)))
+    reps = max(20, a.steps)
+    for _ in range(5):
+        knn.knn_bp_batch_dev(ctx, one, [cfg])
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        knn.knn_bp_batch_dev(ctx, one, [cfg])
+    ctx.sync()
+    ms_one = (time.perf_counter() - t0) * 1e3 / reps
+    knn.knn_bp_batch_dev(ctx, frames, [cfg] * 16)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        knn.knn_bp_batch_dev(ctx, frames, [cfg] * 16)
+    ctx.sync()
+    ms_batch = (time.perf_counter() - t0) * 1e3 / reps
+    gt = synth.fronto_pair(W_, H_, 1)[2]
+    kp0 = probs[0][2]
+    acc = float((got["d1"] == gt[kp0[:, 1], kp0[:, 0]]).mean())
+    out = {
+        "metric": "keypoint-graph BP frames/s, 2000 keypoints x 128 disparities (C3)",
+        "value": round(1e3 / ms_one, 1), "unit": "frames/s",
+        "n_gpus": 1, "steps": reps, "warmup": 5, "ms_per_step": round(ms_one, 4),
+        "batch16_ms": round(ms_batch, 4), "batch16_frames_per_s": round(16e3 / ms_batch, 1),
+        "identical_to_single_call": same, "d1_accuracy_vs_truth": round(acc, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 census / f32 messages",
+        "data": "synthetic (fronto random-dot pair, uniform keypoints, seeds 1..16)",
+        "config": {"workload": "2000 kp, 848x480, census 9x7, 8 labels of d 0..127, k=8 NN, "
+                               "10 iterations, trunc 6"},
+    }
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, ROOT)
+        from oracle import knn_binding as kb
+        L, R, kp = probs[0]
+        c = kb.cfg()
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < min(a.cpu_budget, 10.0):
+            kb.run(L, R, kp, c)
+            n += 1
+        ms = (time.perf_counter() - t0) * 1e3 / n
+        out["cpu_baseline"] = {"value": round(1e3 / ms, 2), "unit": "frames/s", "cores": 1,
+                               "kind": "port", "ms_per_frame": round(ms, 2),
+                               "sample": f"{n} runs of frame 1 (oracle restatement; the "
+                                         "reference has no k-NN BP)"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_c5(a):
+    """C5: serving loop. 16 camera streams, each 30 Hz with Poisson arrivals
+    (exponential inter-arrival, mean 33.3 ms), alternately dense and
+    keypoint-graph, sizes and disparity ranges drawn from the C4 sets with a
+    per-stream seed; dense frames from the slanted-plane generator (census
+    9x7, 8 iterations), keypoint frames from the fronto generator (2000
+    uniform keypoints). Per-stream buffers and scratch resident in HBM,
+    admission by the W x H x D memory budget, continuous batching over 4 HIP
+    streams. Latency = submit to results in host memory."""
+    import numpy as np
+
+    from doubleeye_amd import knn, synth
+    from doubleeye_amd.dense import Cfg, Context
+    from doubleeye_amd.serve import Server
+
+    ctx = Context(0)
+    srv = Server(ctx, mem_budget=16 << 30, hip_streams=4, slots=4)
+    streams = []
+    for k in range(16):
+        rng = np.random.default_rng(500 + k)
+        W_, H_ = C4_SIZES[int(rng.integers(len(C4_SIZES)))]
+        D_ = C4_DISP[int(rng.integers(len(C4_DISP)))]
+        st = dict(kind="dense" if k % 2 == 0 else "knn", W=W_, H=H_, D=D_, frames=[])
+        if st["kind"] == "dense":
+            st["sid"] = srv.open_dense(W_, H_, Cfg(dmax=D_, iters=8, census_hw=4, census_hh=3,
+                                                   sigma_s=20.0, sigma_r=0.0392))
+            for f in range(2):
+                L, R, _ = synth.slanted_pair(W_, H_, seed=1000 * k + f, dhi=min(0.8 * D_, 110.0))
+                st["frames"].append((L, R, None))
+        else:
+            st["sid"] = srv.open_knn(W_, H_, 2000, knn.KnnConfig(dmax=D_ - 1))
+            for f in range(2):
+                L, R, _ = synth.fronto_pair(W_, H_, 1000 * k + f)
+                kp = np.stack([rng.integers(4, W_ - 4, 2000), rng.integers(3, H_ - 3, 2000)],
+                              1).astype(np.int32)
+                st["frames"].append((L, R, kp))
+        streams.append(st)
+    by_sid = {st["sid"]: st for st in streams}
+    # first result of every (stream, frame) is kept to check later ones against
+    first = {}
+    lat = {"dense": [], "knn": []}
+    batches = []
+    mismatches = 0
+    done = 0
+
+    def collect(timeout_ms):
+        nonlocal mismatches, done
+        for r in srv.poll(timeout_ms=timeout_ms):
+            st = by_sid[r.stream_id]
+            if r.status != 0:
+                raise RuntimeError(f"frame failed: status {r.status}")
+            key = (r.stream_id, r.tag % 2)
+            if key not in first or (r.tag // 2) % 25 == 0:   # check a sample
+                out = r.fetch()
+                if key not in first:
+                    first[key] = out
+                else:
+                    ref = first[key]
+                    if st["kind"] == "dense":
+                        ok = all(np.array_equal(x.view(np.uint8), y.view(np.uint8))
+                                 for x, y in zip(out, ref))
+                    else:
+                        ok = np.array_equal(out.view(np.uint8), ref.view(np.uint8))
+                    mismatches += 0 if ok else 1
+            lat[st["kind"]].append(r.latency)
+            batches.append(r.batch_size)
+            srv.release(r)
+            done += 1
+
+    rng = np.random.default_rng(5)
+    count = [0] * 16
+    dropped = 0
+    warm = 1.0
+    t0 = time.perf_counter()
+    nxt = [t0 + rng.exponential(1 / 30.0) for _ in range(16)]
+    submitted_timed = 0
+    pxd = 0
+    last_note = t0
+    while True:
+        if time.perf_counter() - last_note > 10.0:
+            last_note = time.perf_counter()
+            print(f"[c5] {last_note - t0:.0f} s, {done} frames done", file=sys.stderr, flush=True)
+        k = int(np.argmin(nxt))
+        if nxt[k] - t0 > a.duration + warm:
+            break
+        while True:
+            dt = nxt[k] - time.perf_counter()
+            if dt <= 0:
+                break
+            collect(max(0, min(int(dt * 1e3), 2)))
+        st = streams[k]
+        L, R, kp = st["frames"][count[k] % 2]
+        timed = nxt[k] - t0 >= warm
+        if srv.submit(st["sid"], L, R, kp, tag=count[k]) is None:
+            dropped += 1
+        elif timed:
+            submitted_timed += 1
+            if st["kind"] == "dense":
+                pxd += st["W"] * st["H"] * st["D"]
+        if not timed:
+            lat["dense"].clear(); lat["knn"].clear(); batches.clear()
+        count[k] += 1
+        nxt[k] += rng.exponential(1 / 30.0)
+    t_end = time.perf_counter()
+    while done < sum(count) - dropped:
+        collect(50)
+    wall = t_end - t0 - warm
+    alll = lat["dense"] + lat["knn"]
+    pct = lambda xs, p: round(float(np.percentile(xs, p)) * 1e3, 3) if xs else None
+    out = {
+        "metric": "serving p50/p99 frame latency, 16 mixed camera streams at 30 Hz (C5)",
+        "value": pct(alll, 99), "unit": "ms (p99)", "p50_ms": pct(alll, 50),
+        "p99_ms": pct(alll, 99), "max_ms": pct(alll, 100),
+        "dense_p50_ms": pct(lat["dense"], 50), "dense_p99_ms": pct(lat["dense"], 99),
+        "knn_p50_ms": pct(lat["knn"], 50), "knn_p99_ms": pct(lat["knn"], 99),
+        "frames_per_s": round(submitted_timed / wall, 1),
+        "dense_mpx_disp_per_s": round(pxd / wall / 1e6, 1),
+        "frames": submitted_timed, "dropped_busy": dropped,
+        "mean_batch": round(float(np.mean(batches)), 2) if batches else None,
+        "max_batch": int(max(batches)) if batches else None,
+        "sampled_results_identical": mismatches == 0,
+        "n_gpus": 1, "steps": submitted_timed, "warmup": int(warm),
+        "ms_per_step": round(wall * 1e3 / max(1, submitted_timed), 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i16 volume / f32 messages",
+        "data": "synthetic (slanted and fronto random-dot pairs, per-stream seeds)",
+        "config": {"workload": f"16 streams (8 dense, 8 keypoint-graph), Poisson 30 Hz, "
+                               f"{a.duration:.0f} s, sizes/D from the C4 sets",
+                   "streams": [f"{st['kind']} {st['W']}x{st['H']} D{st['D']}" for st in streams]},
+    }
+    srv.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,9 +471,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "sparse"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "sparse"],
                     help="c2: one 848x480x128 pair (default, the headline); c4: 64-pair "
-                         "batch; sparse: keypoint match_masda")
+                         "batch; c3: keypoint-graph BP; c5: serving loop; sparse: keypoint "
+                         "match_masda")
+    ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one plain device-to-device copy of MIB MiB (the "
                          "known-size kernel that scales rocprofv3 FETCH/WRITE_SIZE)")
@@ -272,6 +490,12 @@ def main():
         return
     if a.workload == "sparse":
         bench_sparse(a)
+        return
+    if a.workload == "c3":
+        bench_c3(a)
+        return
+    if a.workload == "c5":
+        bench_c5(a)
         return
 
     import numpy as np

@@ -17,6 +17,7 @@ DE_ERR_ARG = -1
 DE_ERR_UNSUPPORTED = -2
 DE_ERR_HIP = -3
 DE_ERR_NOMEM = -4
+DE_ERR_BUSY = -5
 
 
 class DeError(RuntimeError):
@@ -92,6 +93,21 @@ class DeKnnFrameC(C.Structure):
                 ("kp", C.c_void_p), ("n", C.c_int), ("out", C.c_void_p)]
 
 
+class DeServerCfgC(C.Structure):
+    """de_server_cfg (include/doubleeye_hip.h)."""
+    _fields_ = [("mem_budget", C.c_size_t), ("hip_streams", C.c_int), ("slots", C.c_int),
+                ("max_batch", C.c_int)]
+
+
+class DeServerResultC(C.Structure):
+    """de_server_result (include/doubleeye_hip.h)."""
+    _fields_ = [("stream_id", C.c_int), ("slot", C.c_int), ("kind", C.c_int), ("status", C.c_int),
+                ("tag", C.c_uint64), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_done", C.c_double), ("batch_size", C.c_int), ("W", C.c_int), ("H", C.c_int),
+                ("n", C.c_int), ("disp", C.c_void_p), ("q4", C.c_void_p), ("conf", C.c_void_p),
+                ("knn", C.c_void_p)]
+
+
 # (name, restype, argtypes) for every exported entry point
 P = C.c_void_p
 I = C.c_int
@@ -138,6 +154,16 @@ SIGNATURES = {
     "de_knn_graph": (I, [P, P, P, I, I, P, I, C.POINTER(DeKnnCfgC), P, P, I, C.POINTER(I)]),
     "de_knn_bp_graph": (I, [P, I, P, P, P, P, P, C.POINTER(DeKnnCfgC), P, P]),
     "de_knn_bp_batch_dev": (I, [P, C.POINTER(DeKnnFrameC), C.POINTER(DeKnnCfgC), I, P]),
+    "de_server_default_cfg": (None, [C.POINTER(DeServerCfgC)]),
+    "de_server_create": (I, [P, C.POINTER(DeServerCfgC), C.POINTER(P)]),
+    "de_server_destroy": (None, [P]),
+    "de_server_open_dense": (I, [P, I, I, C.POINTER(DeCfgC), C.POINTER(I)]),
+    "de_server_open_knn": (I, [P, I, I, I, C.POINTER(DeKnnCfgC), C.POINTER(I)]),
+    "de_server_close": (I, [P, I]),
+    "de_server_memory": (I, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "de_server_submit": (I, [P, I, P, P, P, I, C.c_uint64, C.POINTER(I)]),
+    "de_server_poll": (I, [P, C.POINTER(DeServerResultC), I, I, C.POINTER(I)]),
+    "de_server_release": (I, [P, I, I]),
 }
 
 _lib = None

@@ -31,6 +31,7 @@ SOURCES = {
     "de_api.hip": ["-ffp-contract=off"],
     "de_sparse.hip": ["-ffp-contract=off"],
     "de_knn.hip": ["-ffp-contract=off"],
+    "de_serve.hip": [],
 }
 
 

@@ -117,6 +117,13 @@ hipError_t launch_solve(const BatchView& b, int items, size_t lds, hipStream_t s
 hipError_t launch_census_plane(const uint8_t* img, int W, int H, int hw, int hh,
                                uint64_t* out, hipStream_t s);
 
+// keypoint-graph BP over frames whose scratch blocks (knn_scratch_bytes each)
+// the caller provides; launches on s without synchronising
+size_t knn_scratch_bytes(int W, int H, int n);
+int knn_validate(const de_knn_cfg& c, int W, int H, int n);
+int knn_launch_frames(Ctx& x, const de_knn_frame* frames, const de_knn_cfg* cfgs,
+                      void* const* scratch, int n, hipStream_t s);
+
 }  // namespace de
 
 // The opaque context handle of the C ABI.

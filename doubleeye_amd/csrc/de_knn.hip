@@ -688,6 +688,38 @@ static int kb_host(de_ctx* c, const uint8_t* L, const uint8_t* R, int W, int H,
   return DE_OK;
 }
 
+size_t knn_scratch_bytes(int W, int H, int n) { return kb_scratch_bytes(W, H, n, true); }
+
+int knn_validate(const de_knn_cfg& c, int W, int H, int n) { return kb_validate(c, W, H, n); }
+
+int knn_launch_frames(Ctx& x, const de_knn_frame* frames, const de_knn_cfg* cfgs,
+                      void* const* scratch, int n, hipStream_t s) {
+  for (int k = 0; k < n; ++k) {
+    const de_knn_frame& q = frames[k];
+    if (int rc = kb_validate(cfgs[k], q.W, q.H, q.n)) return rc;
+    if (cfgs[k].iters != cfgs[0].iters)
+      return fail(DE_ERR_ARG, "knn batch frame %d: iters must match the first frame's", k);
+    if (q.n > 0 && (!q.L || !q.R || !q.kp || !q.out))
+      return fail(DE_ERR_ARG, "knn batch frame %d: null pointer", k);
+  }
+  std::vector<KbDesc> fr(static_cast<size_t>(n));
+  for (int k = 0; k < n; ++k) {
+    const de_knn_frame& q = frames[k];
+    KbDesc& d = fr[size_t(k)];
+    std::memset(&d, 0, sizeof(d));
+    kb_fill_cfg(cfgs[k], &d);
+    d.L = q.L;
+    d.R = q.R;
+    d.kp = reinterpret_cast<const int2*>(q.kp);
+    d.W = q.W;
+    d.H = q.H;
+    d.n = q.n;
+    d.out = q.out;
+    kb_carve(scratch[k], q.W, q.H, q.n, true, &d);
+  }
+  return kb_run(x, fr, true, cfgs[0].iters, s);
+}
+
 }  // namespace de
 
 using namespace de;
@@ -739,14 +771,6 @@ int de_knn_bp_graph(de_ctx* c, int n, const int32_t* cand_d, const float* cand_c
 int de_knn_bp_batch_dev(de_ctx* c, const de_knn_frame* frames, const de_knn_cfg* cfgs, int n,
                         void* stream) {
   if (!c || !frames || !cfgs || n < 1) return fail(DE_ERR_ARG, "knn batch: %d frames", n);
-  for (int k = 0; k < n; ++k) {
-    const de_knn_frame& q = frames[k];
-    if (int rc = kb_validate(cfgs[k], q.W, q.H, q.n)) return rc;
-    if (cfgs[k].iters != cfgs[0].iters)
-      return fail(DE_ERR_ARG, "knn batch frame %d: iters must match the first frame's", k);
-    if (q.n > 0 && (!q.L || !q.R || !q.kp || !q.out))
-      return fail(DE_ERR_ARG, "knn batch frame %d: null pointer", k);
-  }
   std::lock_guard<std::mutex> lk(c->c.mu);
   Ctx& x = c->c;
   DE_HIP(hipSetDevice(x.device));
@@ -755,25 +779,12 @@ int de_knn_bp_batch_dev(de_ctx* c, const de_knn_frame* frames, const de_knn_cfg*
   size_t total = 0;
   for (int k = 0; k < n; ++k) {
     offs[size_t(k)] = total;
-    total += align256(kb_scratch_bytes(frames[k].W, frames[k].H, frames[k].n, true));
+    total += align256(knn_scratch_bytes(frames[k].W, frames[k].H, frames[k].n));
   }
   if (int rc = x.kb_scratch.ensure(total)) return rc;
-  std::vector<KbDesc> fr(static_cast<size_t>(n));
-  for (int k = 0; k < n; ++k) {
-    const de_knn_frame& q = frames[k];
-    KbDesc& d = fr[size_t(k)];
-    std::memset(&d, 0, sizeof(d));
-    kb_fill_cfg(cfgs[k], &d);
-    d.L = q.L;
-    d.R = q.R;
-    d.kp = reinterpret_cast<const int2*>(q.kp);
-    d.W = q.W;
-    d.H = q.H;
-    d.n = q.n;
-    d.out = q.out;
-    kb_carve(static_cast<uint8_t*>(x.kb_scratch.ptr) + offs[size_t(k)], q.W, q.H, q.n, true, &d);
-  }
-  return kb_run(x, fr, true, cfgs[0].iters, s);
+  std::vector<void*> sc(static_cast<size_t>(n));
+  for (int k = 0; k < n; ++k) sc[size_t(k)] = static_cast<uint8_t*>(x.kb_scratch.ptr) + offs[size_t(k)];
+  return knn_launch_frames(x, frames, cfgs, sc.data(), n, s);
 }
 
 }  // extern "C"

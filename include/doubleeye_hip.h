@@ -24,6 +24,7 @@ extern "C" {
 #define DE_ERR_UNSUPPORTED -2
 #define DE_ERR_HIP -3      /* HIP runtime error; see de_last_error() */
 #define DE_ERR_NOMEM -4
+#define DE_ERR_BUSY -5     /* every buffer slot of a serving stream is in use */
 
 /* Dense configuration: the default-path fields of doubleeye::Cfg
  * (core/include/doubleeye/dense_solve.hpp:19-97) plus the census window.
@@ -320,6 +321,70 @@ typedef struct de_knn_frame {
  * outside the census border get no labels (d1 = -1). */
 int de_knn_bp_batch_dev(de_ctx* ctx, const de_knn_frame* frames, const de_knn_cfg* cfgs,
                         int n, void* stream);
+
+/* ==== Serving loop =========================================================
+ * Many camera streams of mixed geometry on one GPU, with continuous batching.
+ * The reference serves one fixed-geometry stream with one frame in flight
+ * (`de_dense_cuda --stream`, core/tools/de_dense_cuda.cu:670-770, driven by
+ * desktop/de_live_ros2.py). A stream is opened against the server's device
+ * memory budget (its scratch, sized from W x H x D or W x H x keypoints, plus
+ * `slots` input/output buffer sets, all resident for the stream's life).
+ * Submitting copies a frame into a free slot; a scheduler thread batches the
+ * oldest waiting frame of every stream whose previous frame has finished and
+ * launches the batch (copies included) on an idle HIP stream, so batches
+ * overlap; results wait in pinned host memory until released. A frame's
+ * result equals a plain call on the same inputs. Dense streams of one server
+ * share the census window, sigma_s, sigma_r, ad and ad_trunc; while a server
+ * runs, dense calls on its context must use the same values. */
+typedef struct de_server de_server;
+
+typedef struct de_server_cfg {
+  size_t mem_budget;   /* device bytes all open streams may hold together */
+  int hip_streams;     /* batches in flight at once, 1..8 */
+  int slots;           /* buffer sets per camera stream, 1..16 */
+  int max_batch;       /* frames per batch, 0 = no limit */
+} de_server_cfg;
+
+#define DE_STREAM_DENSE 0
+#define DE_STREAM_KNN 1
+
+typedef struct de_server_result {
+  int stream_id, slot, kind, status;
+  uint64_t tag;
+  double t_submit, t_start, t_done;   /* seconds, monotonic clock */
+  int batch_size;                     /* frames launched together */
+  int W, H, n;                        /* geometry; keypoints (knn) */
+  /* host results, valid until de_server_release(stream_id, slot):
+   * dense: disp W*H (NaN = none), q4 W*H (-32768 = none), conf W*H;
+   * knn: knn[n] */
+  const float* disp;
+  const int16_t* q4;
+  const uint8_t* conf;
+  const de_knn_out* knn;
+} de_server_result;
+
+/* 8 GiB, 4 HIP streams, 4 slots, no batch limit. */
+void de_server_default_cfg(de_server_cfg* cfg);
+int de_server_create(de_ctx* ctx, const de_server_cfg* cfg, de_server** out);
+/* Waits for the batches in flight; waiting frames are dropped. */
+void de_server_destroy(de_server* server);
+/* DE_ERR_NOMEM when the stream does not fit the remaining budget. */
+int de_server_open_dense(de_server* server, int W, int H, const de_cfg* cfg, int* stream_id);
+int de_server_open_knn(de_server* server, int W, int H, int max_kp, const de_knn_cfg* cfg,
+                       int* stream_id);
+/* Waits for the stream's frames in flight, drops its waiting frames and
+ * results, returns its memory to the budget. */
+int de_server_close(de_server* server, int stream_id);
+int de_server_memory(de_server* server, size_t* used, size_t* budget);
+/* Host pointers, copied before return. kp/n_kp: knn streams only.
+ * DE_ERR_BUSY when every slot of the stream holds a frame or a result. */
+int de_server_submit(de_server* server, int stream_id, const uint8_t* L, const uint8_t* R,
+                     const int32_t* kp, int n_kp, uint64_t tag, int* slot);
+/* Up to max finished frames, oldest first; waits up to timeout_ms (0: no
+ * wait, < 0: until one is ready). */
+int de_server_poll(de_server* server, de_server_result* results, int max, int timeout_ms,
+                   int* n);
+int de_server_release(de_server* server, int stream_id, int slot);
 
 #ifdef __cplusplus
 }
