@@ -99,15 +99,17 @@ def test_mixed_results_equal_plain_calls(ctx):
                 ("knn", 848, 480, 128, 2), ("dense", 424, 240, 96, 2), ("knn", 320, 240, 64, 2)]
         streams = make_streams(ctx, srv, spec)
         note("[serve] burst submits")
-        n = 0
+        pending, got = 0, []
         for rnd in range(6):
             for st in streams:
                 L, R, kp = st["frames"][rnd % 2]
                 while srv.submit(st["sid"], L, R, kp, tag=rnd) is None:
-                    drain(srv, streams, 1)
-                    n -= 1
-                n += 1
-        got = drain(srv, streams, n)
+                    part = drain(srv, streams, 1)
+                    got += part
+                    pending -= len(part)
+                pending += 1
+        got += drain(srv, streams, pending)
+        assert len(got) == 36
         assert max(r.batch_size for r in got) > 1      # frames were batched together
     finally:
         srv.close()
