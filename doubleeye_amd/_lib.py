@@ -154,6 +154,18 @@ SIGNATURES = {
     "de_knn_graph": (I, [P, P, P, I, I, P, I, C.POINTER(DeKnnCfgC), P, P, I, C.POINTER(I)]),
     "de_knn_bp_graph": (I, [P, I, P, P, P, P, P, C.POINTER(DeKnnCfgC), P, P]),
     "de_knn_bp_batch_dev": (I, [P, C.POINTER(DeKnnFrameC), C.POINTER(DeKnnCfgC), I, P]),
+    "de_dense_problem": (I, [P, P, P, I, I, I, I, I, I, P, P, P, C.c_int64,
+                             C.POINTER(C.c_int64)]),
+    "de_masda_edges": (I, [P, P, P, P, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                           I, C.c_double, C.c_double, P, P, C.POINTER(I)]),
+    "de_masda_edges_f32": (I, [P, P, P, P, C.c_int64, C.c_int64, C.c_int64, C.c_double,
+                               C.c_double, I, C.c_double, C.c_double, P, P, C.POINTER(I)]),
+    "de_masda_matrix": (I, [P, P, C.c_int64, C.c_int64, C.c_double, C.c_double, I, C.c_double,
+                            C.c_double, P, P, C.POINTER(I)]),
+    "de_masda_decide": (I, [P, P, P, P, P, C.c_int64, C.c_int64, C.c_int64, C.c_double, P]),
+    "de_seg_max_excluding": (I, [P, P, P, C.c_int64, C.c_int64, P]),
+    "de_dense_masda": (I, [P, P, P, I, I, I, I, I, I, C.c_double, C.c_double, I, C.c_double,
+                           C.c_double, P, P, C.POINTER(I)]),
     "de_server_default_cfg": (None, [C.POINTER(DeServerCfgC)]),
     "de_server_create": (I, [P, C.POINTER(DeServerCfgC), C.POINTER(P)]),
     "de_server_destroy": (None, [P]),

@@ -32,6 +32,7 @@ SOURCES = {
     "de_sparse.hip": ["-ffp-contract=off"],
     "de_knn.hip": ["-ffp-contract=off"],
     "de_serve.hip": [],
+    "de_edges.hip": ["-ffp-contract=off"],
 }
 
 

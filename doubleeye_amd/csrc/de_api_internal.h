@@ -59,6 +59,7 @@ struct Ctx {
   DevBuf scratch, rf_buf, sc_buf, io_in, io_out;
   DevBuf sp_scratch, sp_io;   // sparse matcher: per-frame scratch, staged inputs/outputs
   DevBuf kb_scratch, kb_io;   // keypoint-graph BP: scratch, staged inputs/outputs
+  DevBuf ed_scratch, ed_io;   // edge-list MASDA: scratch, staged inputs/outputs
   Slot slots[kSlots];
   int next_slot = 0;
   bool store_vol = false;

@@ -322,6 +322,56 @@ typedef struct de_knn_frame {
 int de_knn_bp_batch_dev(de_ctx* ctx, const de_knn_frame* frames, const de_knn_cfg* cfgs,
                         int n, void* stream);
 
+/* ==== Edge-list MASDA in float64: the NumPy call shapes =====================
+ * article/masda_stereo.py and article/dense_masda.py. Messages and beliefs in
+ * float64 with numpy's rounding; the greedy decision takes edges in (belief
+ * desc, edge index asc) order, where numpy's unstable argsort may order exact
+ * ties differently. Host pointers; index arrays are int64 like numpy's. */
+
+/* dense_problem (dense_masda.py:39-79): every interior pixel against every
+ * candidate d in [dmin, dmax] on its row; edges d-major, then y, then x; left
+ * index y*W + x, right index y*W + x - d, score (24 - Hamming) / 24 (float32).
+ * With ei, ej and se all NULL only *n_edges is computed. The node count is W*H. */
+int de_dense_problem(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+                     int dmin, int dmax, int census_hw, int census_hh, int64_t* ei,
+                     int64_t* ej, float* se, int64_t capacity, int64_t* n_edges);
+
+/* masda_sparse (masda_stereo.py:453-492). assign[i] = j or -1 for the m left
+ * nodes; hist gets the largest message change of each iteration run
+ * (*n_iter of them, at most iters). The _f32 form takes float32 scores (as
+ * dense_problem returns them); they enter the float64 arithmetic exactly. */
+int de_masda_edges(de_ctx* ctx, const int64_t* ei, const int64_t* ej, const double* se,
+                   int64_t n_edges, int64_t m, int64_t n, double lam, double gam, int iters,
+                   double damping, double eps, int64_t* assign, double* hist, int* n_iter);
+int de_masda_edges_f32(de_ctx* ctx, const int64_t* ei, const int64_t* ej, const float* se,
+                       int64_t n_edges, int64_t m, int64_t n, double lam, double gam,
+                       int iters, double damping, double eps, int64_t* assign, double* hist,
+                       int* n_iter);
+
+/* masda (masda_stereo.py:329-389): a dense m x n float32 score matrix, -inf
+ * where there is no candidate; same solver on its row-major edges. */
+int de_masda_matrix(de_ctx* ctx, const float* S, int64_t m, int64_t n, double lam,
+                    double gam, int iters, double damping, double eps, int64_t* assign,
+                    double* hist, int* n_iter);
+
+/* decide (masda_stereo.py:392-414) on an edge list: greedy in belief order,
+ * an edge eligible when its score beats lam and its belief is finite. */
+int de_masda_decide(de_ctx* ctx, const int64_t* ei, const int64_t* ej, const double* se,
+                    const double* belief, int64_t n_edges, int64_t m, int64_t n, double lam,
+                    int64_t* assign);
+
+/* _seg_max_excluding (masda_stereo.py:417-442): out[e] = max over the other
+ * elements of segment idx[e] (-inf when it is alone). */
+int de_seg_max_excluding(de_ctx* ctx, const double* vals, const int64_t* idx, int64_t n_vals,
+                         int64_t n_segments, double* out);
+
+/* dense_problem + masda_sparse + the disparity map of dense_masda.run
+ * (dense_masda.py:82-110, without the optional margin gate), all on the
+ * device: disp[y*W + x] = x - (j mod W) for matched pixels, NaN elsewhere. */
+int de_dense_masda(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H, int dmin,
+                   int dmax, int census_hw, int census_hh, double lam, double gam, int iters,
+                   double damping, double eps, float* disp, double* hist, int* n_iter);
+
 /* ==== Serving loop =========================================================
  * Many camera streams of mixed geometry on one GPU, with continuous batching.
  * The reference serves one fixed-geometry stream with one frame in flight

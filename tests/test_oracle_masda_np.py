@@ -1,0 +1,115 @@
+"""The NumPy MASDA oracle (oracle/masda_np.py) against vectors written from
+the reference's own article/masda_stereo.py (tests/golden/masda_np.npz), an
+enumeration of _seg_max_excluding, an enumeration of dense_problem's edges,
+and exhaustive optimal assignments. CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+
+import masda_np_cases as mc
+from conftest import GOLDEN  # noqa: F401
+
+SP, MX, DC = mc.load()
+
+
+def test_seg_max_excluding_by_enumeration(oracle):
+    from oracle import masda_np as mn
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        E, n = int(rng.integers(1, 30)), int(rng.integers(1, 8))
+        idx = rng.integers(0, n, E)
+        vals = rng.integers(-3, 4, E).astype(np.float64)   # many ties
+        got = mn.seg_max_excluding(vals, idx, n)
+        for e in range(E):
+            others = [vals[f] for f in range(E) if f != e and idx[f] == idx[e]]
+            assert got[e] == (max(others) if others else -np.inf)
+
+
+@pytest.mark.parametrize("k", range(len(SP)))
+def test_masda_sparse_equals_reference(k, oracle):
+    from oracle import masda_np as mn
+    p = SP[k]
+    a, hist, bel = mn.masda_sparse(p["ei"], p["ej"], p["se"], p["m"], p["n"], canonical=False,
+                                   return_belief=True, **p["kw"])
+    assert np.array_equal(hist, p["hist"])
+    assert np.array_equal(mc.pairs(a), p["assign"])           # same dict, same order
+    c, _ = mn.masda_sparse(p["ei"], p["ej"], p["se"], p["m"], p["n"], canonical=True, **p["kw"])
+    if not mc.has_belief_ties(bel):
+        assert np.array_equal(mc.pairs(c), p["assign"])
+
+
+@pytest.mark.parametrize("k", range(len(MX)))
+def test_masda_matrix_equals_reference(k, oracle):
+    from oracle import masda_np as mn
+    p = MX[k]
+    a, hist = mn.masda(p["S"], canonical=False, **p["kw"])
+    assert np.array_equal(hist, p["hist"])
+    assert np.array_equal(mc.pairs(a), p["assign"])
+
+
+@pytest.mark.parametrize("k", range(len(DC)))
+def test_decide_equals_reference(k, oracle):
+    from oracle import masda_np as mn
+    p = DC[k]
+    assert np.array_equal(mc.pairs(mn.decide(p["S"], p["bel"], p["lam"], canonical=False)),
+                          p["assign"])
+    assert np.array_equal(mc.pairs(mn.decide(p["S"], p["bel"], p["lam"], canonical=True)),
+                          p["assign"])   # normal beliefs: no exact ties
+
+
+def test_dense_problem_by_enumeration(oracle):
+    """Edges, their order and scores from the definition, with a census
+    computed here pixel by pixel."""
+    from doubleeye_amd import synth
+    from oracle import masda_np as mn
+    L, R, _ = synth.fronto_pair(40, 16, 3)
+    ei, ej, se, nodes = mn.dense_problem(L, R, 1, 6)
+    H, W = L.shape
+    assert nodes == H * W
+
+    def census(img, x, y):
+        b, v = 0, 0
+        for dy in range(-3, 4):
+            for dx in range(-3, 4):
+                if dx or dy:
+                    v |= int(img[y + dy, x + dx] < img[y, x]) << b
+                    b += 1
+        return v
+
+    k = 0
+    for d in range(1, 7):
+        for y in range(3, H - 3):
+            for x in range(3 + d, W - 3):
+                h = bin(census(L, x, y) ^ census(R, x - d, y)).count("1")
+                assert (ei[k], ej[k]) == (y * W + x, y * W + x - d)
+                assert se[k] == np.float32((24.0 - h) / 24.0)
+                k += 1
+    assert k == len(se)
+
+
+def test_masda_sparse_finds_the_optimum_on_tiny_problems(oracle):
+    """Exhaustive maximum-weight matching (lambda per unmatched left, gamma per
+    unmatched right): MASDA reaches it on the large majority of problems."""
+    from oracle import masda_np as mn
+    rng = np.random.default_rng(4)
+    hit = total = 0
+    for _ in range(60):
+        m, n = int(rng.integers(2, 6)), int(rng.integers(2, 6))
+        S = np.where(rng.random((m, n)) < 0.7, rng.uniform(-0.5, 1, (m, n)), -np.inf)
+        ei, ej = np.nonzero(np.isfinite(S))
+        a, _ = mn.masda_sparse(ei, ej, S[ei, ej], m, n, lam=-0.1, gam=-0.1, iters=60)
+
+        def value(asg):
+            return sum(S[i, j] for i, j in asg.items()) - 0.1 * (m - len(asg)) - 0.1 * (n - len(asg))
+
+        best = -np.inf
+        for perm in itertools.product(*[[-1] + list(range(n))] * m):
+            used = [j for j in perm if j >= 0]
+            if len(set(used)) < len(used) or any(j >= 0 and not np.isfinite(S[i, j])
+                                                 for i, j in enumerate(perm)):
+                continue
+            best = max(best, value({i: j for i, j in enumerate(perm) if j >= 0}))
+        total += 1
+        hit += value(a) >= best - 1e-9
+    assert hit >= 0.9 * total, (hit, total)
