@@ -464,6 +464,57 @@ def bench_c5(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_dense_masda(a):
+    """The NumPy dense MASDA of the reference's article (dense_masda.run,
+    dense_masda.py:82-110) at its Middlebury size: 450x375, d 1..60, census 7x7,
+    12 iterations, lambda = gamma = -0.1 (10.1 M edges). The device pipeline
+    (edge list, float64 messages, greedy decision, disparity map) from host
+    images to a host disparity map. The CPU baseline is the oracle's NumPy
+    restatement of the same functions, single thread, one run."""
+    import numpy as np
+
+    from doubleeye_amd import masda_stereo as gm
+    from doubleeye_amd import synth
+    from doubleeye_amd.dense import Context
+
+    ctx = Context(0)
+    W_, H_ = 450, 375
+    L, R, gt = synth.fronto_pair(W_, H_, 9)
+    disp, hist = gm.dense_masda(L, R, 1, 60, iters=12, ctx=ctx)
+    reps = max(3, a.steps // 40)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        disp, hist = gm.dense_masda(L, R, 1, 60, iters=12, ctx=ctx)
+    ms_ = (time.perf_counter() - t0) * 1e3 / reps
+    E = 375 - 6
+    edges = sum(max(0, 450 - 6 - d) for d in range(1, 61)) * E
+    ok = np.isfinite(disp)
+    out = {
+        "metric": "NumPy-shape dense MASDA (dense_masda.run) time, 450x375, d 1..60, 12 it",
+        "value": round(ms_, 3), "unit": "ms", "edges": edges,
+        "edges_per_s": round(edges * len(hist) / (ms_ * 1e-3), 1),
+        "iterations_run": len(hist), "coverage": round(float(ok.mean()), 4),
+        "bad_1px_vs_truth": round(float((np.abs(disp[ok] - gt[ok]) > 1.0).mean()), 4),
+        "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": round(ms_, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 messages", "data": "synthetic (fronto random-dot pair 450x375, seed 9)",
+        "config": {"workload": "dense_problem + masda_sparse + disparity map, host to host"},
+    }
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, ROOT)
+        from oracle import masda_np as mn
+        t0 = time.perf_counter()
+        ei, ej, se, n = mn.dense_problem(L, R, 1, 60)
+        oa, oh = mn.masda_sparse(ei, ej, se, n, n, -0.1, -0.1, iters=12)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(t * 1e3, 1), "unit": "ms", "cores": 1,
+                               "kind": "port", "sample": "one run of the same frame (NumPy "
+                               "restatement of dense_problem + masda_sparse)",
+                               "same_history": bool(np.array_equal(oh, hist))}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -471,7 +522,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "sparse"],
+    ap.add_argument("--workload", default="c2",
+                    choices=["c2", "c3", "c4", "c5", "sparse", "dense_masda"],
                     help="c2: one 848x480x128 pair (default, the headline); c4: 64-pair "
                          "batch; c3: keypoint-graph BP; c5: serving loop; sparse: keypoint "
                          "match_masda")
@@ -496,6 +548,9 @@ def main():
         return
     if a.workload == "c5":
         bench_c5(a)
+        return
+    if a.workload == "dense_masda":
+        bench_dense_masda(a)
         return
 
     import numpy as np

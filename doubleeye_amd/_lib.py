@@ -154,6 +154,9 @@ SIGNATURES = {
     "de_knn_graph": (I, [P, P, P, I, I, P, I, C.POINTER(DeKnnCfgC), P, P, I, C.POINTER(I)]),
     "de_knn_bp_graph": (I, [P, I, P, P, P, P, P, C.POINTER(DeKnnCfgC), P, P]),
     "de_knn_bp_batch_dev": (I, [P, C.POINTER(DeKnnFrameC), C.POINTER(DeKnnCfgC), I, P]),
+    "de_dense_volume_f32": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P]),
+    "de_dense_run_full": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, P]),
+    "de_dense_run_full_dev": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, P, P]),
     "de_dense_problem": (I, [P, P, P, I, I, I, I, I, I, P, P, P, C.c_int64,
                              C.POINTER(C.c_int64)]),
     "de_masda_edges": (I, [P, P, P, P, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_double,

@@ -33,6 +33,7 @@ SOURCES = {
     "de_knn.hip": ["-ffp-contract=off"],
     "de_serve.hip": [],
     "de_edges.hip": ["-ffp-contract=off"],
+    "de_fp32.hip": ["-ffp-contract=off"],
 }
 
 

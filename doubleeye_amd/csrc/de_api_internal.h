@@ -60,6 +60,7 @@ struct Ctx {
   DevBuf sp_scratch, sp_io;   // sparse matcher: per-frame scratch, staged inputs/outputs
   DevBuf kb_scratch, kb_io;   // keypoint-graph BP: scratch, staged inputs/outputs
   DevBuf ed_scratch, ed_io;   // edge-list MASDA: scratch, staged inputs/outputs
+  DevBuf f32_scratch, f32_io; // FP32 full-volume path: scratch, staged inputs/outputs
   Slot slots[kSlots];
   int next_slot = 0;
   bool store_vol = false;

@@ -217,6 +217,27 @@ class Context:
         check(lib().de_dense_volume(self.h, _p(L), _p(R), W, H, C.byref(c), k0, k1, _p(out)))
         return out
 
+    def volume_f32(self, L, R, cfg: Cfg) -> np.ndarray:
+        """The FP32 aggregated volume [y][x][k] (`de_dense --dump-vol`)."""
+        L = np.ascontiguousarray(L, np.uint8)
+        R = np.ascontiguousarray(R, np.uint8)
+        H, W = L.shape
+        out = np.empty((H, W, cfg.D), np.float32)
+        check(lib().de_dense_volume_f32(self.h, _p(L), _p(R), W, H, C.byref(cfg.to_c()), _p(out)))
+        return out
+
+    def run_dense_full(self, L, R, cfg: Cfg):
+        """`de_dense --topk 0`: FP32 volume + full-volume row solver ->
+        (disp, margin)."""
+        L = np.ascontiguousarray(L, np.uint8)
+        R = np.ascontiguousarray(R, np.uint8)
+        H, W = L.shape
+        disp = np.empty((H, W), np.float32)
+        margin = np.empty((H, W), np.float32)
+        check(lib().de_dense_run_full(self.h, _p(L), _p(R), W, H, C.byref(cfg.to_c()), _p(disp),
+                                      _p(margin)))
+        return disp, margin
+
     def candidates(self, L, R, cfg: Cfg) -> np.ndarray:
         L = np.ascontiguousarray(L, np.uint8)
         R = np.ascontiguousarray(R, np.uint8)

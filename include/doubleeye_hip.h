@@ -101,6 +101,21 @@ int de_dense_run_dev(de_ctx* ctx, const uint8_t* d_L, const uint8_t* d_R, int W,
                      int H, const de_cfg* cfg, float* d_disp, float* d_margin,
                      float* d_cand6, void* stream);
 
+/* The FP32 full-volume path (`de_dense --no-blockwise/--dump-vol`): the float
+ * score and recursive filter (core/tools/de_dense.cpp:379-396, 510-527,
+ * 1305-1318), volume [y][x][k] of W*H*(dmax-dmin+1) floats, -1e30 outside each
+ * pixel's valid window, exactly the --dump-vol file (de_dense.cpp:1433-1436).
+ * Host pointers. */
+int de_dense_volume_f32(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+                        const de_cfg* cfg, float* vol);
+
+/* `de_dense --topk 0`: the FP32 volume and the full-volume row solver
+ * solve_row (de_dense.cpp:566-712). disp (NaN = no answer) and margin, W*H. */
+int de_dense_run_full(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+                      const de_cfg* cfg, float* disp, float* margin);
+int de_dense_run_full_dev(de_ctx* ctx, const uint8_t* d_L, const uint8_t* d_R, int W, int H,
+                          const de_cfg* cfg, float* d_disp, float* d_margin, void* stream);
+
 /* ---- Batches: many pairs of mixed size and disparity range, one launch per
  * stage (plus one column-pass launch per disparity-group count). No
  * counterpart in the reference, which runs one fixed-geometry pair at a time

@@ -54,6 +54,7 @@ def lib():
                                               C.c_int, C.c_int, P]
         L.orc_dense_run.argtypes = [P, P, C.c_int, C.c_int, C.POINTER(OrcCfg), P, P, P, P]
         L.orc_volume_f32.argtypes = [P, P, C.c_int, C.c_int, C.POINTER(OrcCfg), P]
+        L.orc_dense_run_full.argtypes = [P, P, C.c_int, C.c_int, C.POINTER(OrcCfg), P, P, P]
         L.orc_confidence.argtypes = [C.c_float] * 5
         L.orc_confidence.restype = C.c_float
         L.orc_confidence_u8.argtypes = [C.c_float] * 5
@@ -134,6 +135,21 @@ def dense_run(L, R, c: OrcCfg, want_cand: bool = False, want_ties: bool = False)
     if rc:
         raise ValueError(f"orc_dense_run rc={rc}")
     return disp, margin, cand, ties
+
+
+def dense_run_full(L, R, c: OrcCfg, want_ties: bool = False):
+    """--topk 0: (disp, margin, tie_mask or None)."""
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    disp = np.empty((H, W), np.float32)
+    margin = np.empty((H, W), np.float32)
+    ties = np.zeros((H, W), np.uint8) if want_ties else None
+    rc = lib().orc_dense_run_full(_p(L), _p(R), W, H, C.byref(c), _p(disp), _p(margin),
+                                  _p(ties) if ties is not None else None)
+    if rc:
+        raise ValueError(f"orc_dense_run_full rc={rc}")
+    return disp, margin, ties
 
 
 def volume_f32(L, R, c: OrcCfg) -> np.ndarray:

@@ -550,6 +550,153 @@ int orc_volume_f32(const uint8_t* L, const uint8_t* R, int W, int H,
   return 0;
 }
 
+// Full-volume row solver (--topk 0): scores over each pixel's whole valid
+// disparity interval, `iters` rounds of rho (over a left pixel's disparities)
+// and beta (over a right pixel's claimants, the diagonal x = xr + dmin + k),
+// best belief per pixel, greedy claim in std::sort order, parabola sub-pixel.
+// Follows core/tools/de_dense.cpp:566-712; the caller passes hw = 3 there
+// (de_dense.cpp:1541), whatever the census window.
+static void solve_row_full(int W, int D, const OrcCfg& cfg, const float* vol, float* s,
+                           float* beta, float* rho, float* out_disp, float* out_margin,
+                           int* tie_flag) {
+  const int hw = 3;
+  const int dmin = cfg.dmin;
+  const float keep = 1.f - cfg.damping;
+  std::vector<int> k0(W), k1(W), bestk(W), order(W);
+  std::vector<float> best(W);
+  std::vector<char> taken(W);
+  std::vector<float> gath(size_t(D) + 2), sv(size_t(D) + 2);
+  std::vector<int> ok(size_t(D) + 2);
+  for (int x = 0; x < W; ++x) {
+    int lo = 0, hi = 0;
+    if (x >= hw && x < W - hw) {
+      lo = std::max(0, x - (W - hw) + 1 - dmin + 1);
+      hi = std::min(D, x - hw - dmin + 1);
+      if (hi < lo) hi = lo;
+    }
+    k0[x] = lo;
+    k1[x] = hi;
+    Top2f t;
+    for (int k = lo; k < hi; ++k) {
+      const float v = vol[size_t(x) * D + k];
+      s[size_t(x) * D + k] = v;
+      if (v > -1e29f) t.push(v, k);
+    }
+    const float alt = (t.b2 < -1e29f) ? cfg.lambda : std::max(t.b2, cfg.lambda);
+    out_margin[x] = (t.i1 < 0) ? 0.f : t.b1 - alt;
+  }
+  std::fill(beta, beta + size_t(W) * D, 0.f);
+  std::fill(rho, rho + size_t(W) * D, 0.f);
+  for (int it = 0; it < cfg.iters; ++it) {
+    for (int x = 0; x < W; ++x) {
+      const int lo = k0[x], hi = k1[x];
+      if (hi <= lo) continue;
+      Top2f t;
+      for (int k = lo; k < hi; ++k) t.push(beta[size_t(x) * D + k], k);
+      for (int k = lo; k < hi; ++k) {
+        const size_t i = size_t(x) * D + k;
+        const float tgt = s[i] - std::max(cfg.lambda, t.excl(k));
+        rho[i] = keep * tgt + cfg.damping * rho[i];
+      }
+    }
+    for (int xr = 0; xr < W; ++xr) {
+      const int kmax = std::min(D, W - xr - dmin);
+      if (kmax <= 0) break;
+      Top2f t;
+      int nv = 0;
+      for (int k = 0; k < kmax; ++k) {
+        const int x = xr + dmin + k;
+        ok[size_t(k)] = (k >= k0[x] && k < k1[x]);
+        if (!ok[size_t(k)]) continue;
+        const size_t i = size_t(x) * D + k;
+        gath[size_t(k)] = rho[i];
+        sv[size_t(k)] = s[i];
+        t.push(gath[size_t(k)], k);
+        ++nv;
+      }
+      if (nv == 0) continue;
+      for (int k = 0; k < kmax; ++k) {
+        if (!ok[size_t(k)]) continue;
+        const size_t i = size_t(xr + dmin + k) * D + k;
+        beta[i] = keep * (sv[size_t(k)] - std::max(cfg.gamma, t.excl(k))) + cfg.damping * beta[i];
+      }
+    }
+  }
+  for (int x = 0; x < W; ++x) {
+    best[size_t(x)] = -1e30f;
+    bestk[size_t(x)] = -1;
+    taken[size_t(x)] = 0;
+    order[size_t(x)] = x;
+    for (int k = k0[x]; k < k1[x]; ++k) {
+      const size_t i = size_t(x) * D + k;
+      if (s[i] <= cfg.lambda) continue;
+      const float bel = beta[i] + rho[i] - s[i];
+      if (bel > best[size_t(x)]) { best[size_t(x)] = bel; bestk[size_t(x)] = k; }
+    }
+  }
+  if (tie_flag) {
+    std::vector<float> top(W, kNone);
+    std::vector<int> cnt(W, 0);
+    for (int x = 0; x < W; ++x) {
+      tie_flag[x] = 0;
+      if (bestk[size_t(x)] < 0) continue;
+      const int xr = x - (dmin + bestk[size_t(x)]);
+      if (xr < 0) continue;
+      if (best[size_t(x)] > top[size_t(xr)]) { top[size_t(xr)] = best[size_t(x)]; cnt[size_t(xr)] = 1; }
+      else if (best[size_t(x)] == top[size_t(xr)]) ++cnt[size_t(xr)];
+    }
+    for (int x = 0; x < W; ++x) {
+      if (bestk[size_t(x)] < 0) continue;
+      const int xr = x - (dmin + bestk[size_t(x)]);
+      if (xr >= 0 && cnt[size_t(xr)] > 1 && best[size_t(x)] == top[size_t(xr)]) tie_flag[x] = 1;
+    }
+  }
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return best[size_t(a)] > best[size_t(b)]; });
+  for (int oi = 0; oi < W; ++oi) {
+    const int x = order[size_t(oi)];
+    out_disp[x] = std::nanf("");
+    if (bestk[size_t(x)] < 0) continue;
+    if (cfg.min_margin > 0.f && out_margin[x] < cfg.min_margin) continue;
+    const int k = bestk[size_t(x)];
+    const int d = dmin + k;
+    const int xr = x - d;
+    if (xr < 0 || taken[size_t(xr)]) continue;
+    taken[size_t(xr)] = 1;
+    float off = 0.f;
+    const float* sp = s + size_t(x) * D;
+    if (cfg.subpixel && k > k0[x] && k + 1 < k1[x]) {
+      const float cm = sp[k - 1], c0 = sp[k], cp = sp[k + 1];
+      const float den = 2.f * c0 - cm - cp;
+      if (den > 1e-6f) {
+        off = 0.5f * (cp - cm) / den;
+        if (off > 0.5f) off = 0.5f;
+        if (off < -0.5f) off = -0.5f;
+      }
+    }
+    out_disp[x] = float(d) + off;
+  }
+}
+
+// --topk 0: FP32 volume + full-volume row solver. tie_mask as orc_dense_run.
+int orc_dense_run_full(const uint8_t* L, const uint8_t* R, int W, int H, const OrcCfg* cfgp,
+                       float* disp, float* margin, uint8_t* tie_mask) {
+  if (!L || !R || !cfgp || !disp || !margin || W <= 0 || H <= 0) return -1;
+  const OrcCfg& cfg = *cfgp;
+  const int D = cfg.dmax - cfg.dmin + 1;
+  if (D < 1) return -3;
+  std::vector<float> vol(size_t(W) * H * D);
+  if (int rc = orc_volume_f32(L, R, W, H, cfgp, vol.data())) return rc;
+  std::vector<float> s(size_t(W) * D), beta(size_t(W) * D), rho(size_t(W) * D);
+  std::vector<int> tie(W);
+  for (int y = 0; y < H; ++y) {
+    solve_row_full(W, D, cfg, &vol[size_t(y) * W * D], s.data(), beta.data(), rho.data(),
+                   &disp[size_t(y) * W], &margin[size_t(y) * W], tie_mask ? tie.data() : nullptr);
+    if (tie_mask)
+      for (int x = 0; x < W; ++x) tie_mask[size_t(y) * W + x] = uint8_t(tie[size_t(x)]);
+  }
+  return 0;
+}
+
 // dense_solve.hpp:158-171.
 float orc_confidence(float s1, float s2, float lambda, float lrc, float lrc_tol) {
   if (!(s1 > -1e29f)) return 0.f;

@@ -32,6 +32,8 @@ int orc_filtered_planes_i16(const uint8_t* L, const uint8_t* R, int W, int H,
 int orc_dense_run(const uint8_t* L, const uint8_t* R, int W, int H,
                   const OrcCfg* cfg, float* disp, float* margin, float* cand6,
                   uint8_t* tie_mask);
+int orc_dense_run_full(const uint8_t* L, const uint8_t* R, int W, int H, const OrcCfg* cfg,
+                       float* disp, float* margin, uint8_t* tie_mask);
 int orc_volume_f32(const uint8_t* L, const uint8_t* R, int W, int H,
                    const OrcCfg* cfg, float* vol);
 float orc_confidence(float s1, float s2, float lambda, float lrc, float lrc_tol);
