@@ -155,9 +155,6 @@ int validate(const de_cfg& c, int W, int H) {
   if (c.iters < 0) return fail(DE_ERR_ARG, "iters %d < 0", c.iters);
   if (!(c.sigma_s > 0.f) || !(c.sigma_r > 0.f))
     return fail(DE_ERR_ARG, "sigma_s and sigma_r must be positive");
-  if (c.lrc)
-    return fail(DE_ERR_UNSUPPORTED,
-                "exact reverse match (--lrc) is not built yet; lrc2 is always written");
   return DE_OK;
 }
 
@@ -178,8 +175,11 @@ int Ctx::frame_desc(const de_cfg& c, int W, int H, const uint8_t* L, const uint8
   fd->W = W; fd->H = H; fd->D = D; fd->G = (D + kWave - 1) / kWave; fd->Dpad = fd->G * kWave;
   fd->dmin = c.dmin; fd->hw = c.census_hw; fd->hh = c.census_hh;
   fd->sc_shift = shift; fd->ad_T = std::min(255, std::max(1, c.ad_trunc));
-  fd->store_vol = store_vol ? 1 : 0;
-  fd->iters = c.iters; fd->subpixel = c.subpixel; fd->fit_eq = c.fit_eq; fd->lrc = c.lrc;
+  // the exact reverse match reads the final volume, so the column pass stores it
+  fd->lrc = (c.lrc && cand6) ? 1 : 0;
+  fd->store_vol = (store_vol || fd->lrc) ? 1 : 0;
+  fd->rdisp = sp.rdisp;
+  fd->iters = c.iters; fd->subpixel = c.subpixel; fd->fit_eq = c.fit_eq;
   fd->lambda = c.lambda; fd->gamma = c.gamma; fd->damping = c.damping;
   fd->min_margin = c.min_margin; fd->min_ratio = c.min_ratio;
   return DE_OK;
@@ -189,7 +189,7 @@ size_t scratch_bytes(int W, int H, int D) {
   const size_t WH = size_t(W) * H;
   const int Dpad = ((D + kWave - 1) / kWave) * kWave;
   const size_t vol_rows = size_t(H) + 2 * kVolPadRows;
-  return WH * (16 + 8 + 4 + 16) + vol_rows * W * size_t(Dpad) * 2 + 5 * 256;
+  return WH * (16 + 8 + 4 + 16 + 2) + vol_rows * W * size_t(Dpad) * 2 + 6 * 256;
 }
 
 // Carve one frame's scratch out of a device block.
@@ -203,6 +203,7 @@ Scratch carve_scratch(void* base, int W, int H, int D) {
   s.cr = reinterpret_cast<uint64_t*>(take(WH * 8));
   s.ayt = reinterpret_cast<uint32_t*>(take(WH * 4));
   s.cand = reinterpret_cast<uint32_t*>(take(WH * 16));
+  s.rdisp = reinterpret_cast<int16_t*>(take(WH * 2));
   // the volume keeps kVolPadRows rows of padding above and below the frame
   const size_t row_elems = size_t(W) * Dpad;
   uint16_t* v = reinterpret_cast<uint16_t*>(take((size_t(H) + 2 * kVolPadRows) * row_elems * 2));
@@ -274,14 +275,24 @@ int Ctx::launch_stages(const Prepared& P, const BatchView& bv, hipStream_t s, in
       while (j < n && fds[j].G == fds[i].G) ++j;
       const int base = P.start[size_t(kStageCol) * (n + 1) + i];
       const int cnt = P.start[size_t(kStageCol) * (n + 1) + j] - base;
-      if ((e = launch_col(bv, base, cnt, fds[i].G, fds[i].store_vol != 0, P.colv, s)) != hipSuccess)
+      bool store = false;
+      for (int q = i; q < j; ++q) store = store || fds[q].store_vol != 0;
+      if ((e = launch_col(bv, base, cnt, fds[i].G, store, P.colv, s)) != hipSuccess)
         return hip_fail(e, "k_col");
       i = j;
     }
   }
+  bool any_lrc = false;
+  for (int q = 0; q < n; ++q) any_lrc = any_lrc || fds[q].lrc != 0;
+  if ((stage_mask & 4) && any_lrc &&
+      (e = launch_lrc_exact(bv, items(kStageSolve), s)) != hipSuccess)
+    return hip_fail(e, "k_lrc_exact");
   if (ev) DE_HIP(hipEventRecord(ev->e[3], s));
   if ((stage_mask & 8) && (e = launch_solve(bv, items(kStageSolve), P.lds_solve, s)) != hipSuccess)
     return hip_fail(e, "k_solve");
+  if ((stage_mask & 8) && any_lrc &&
+      (e = launch_lrc_plane(bv, items(kStageSolve), s)) != hipSuccess)
+    return hip_fail(e, "k_lrc_plane");
   if (ev) DE_HIP(hipEventRecord(ev->e[4], s));
   return DE_OK;
 }

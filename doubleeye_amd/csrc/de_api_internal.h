@@ -40,6 +40,7 @@ struct Scratch {
   uint32_t* ayt;
   uint32_t* cand;
   uint16_t* vol;
+  int16_t* rdisp;
 };
 size_t scratch_bytes(int W, int H, int D);
 Scratch carve_scratch(void* base, int W, int H, int D);
@@ -116,6 +117,8 @@ hipError_t launch_row(const BatchView& b, int items, size_t lds, int variant, hi
 hipError_t launch_col(const BatchView& b, int base, int items, int G, bool store, int variant,
                       hipStream_t s);
 hipError_t launch_solve(const BatchView& b, int items, size_t lds, hipStream_t s);
+hipError_t launch_lrc_exact(const BatchView& b, int items, hipStream_t s);
+hipError_t launch_lrc_plane(const BatchView& b, int items, hipStream_t s);
 hipError_t launch_census_plane(const uint8_t* img, int W, int H, int hw, int hh,
                                uint64_t* out, hipStream_t s);
 

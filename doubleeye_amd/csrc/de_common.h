@@ -55,6 +55,7 @@ struct FrameDesc {
   uint32_t* ayt;         // 2*ay, transposed [x][y] (scalar loads down a column)
   uint16_t* vol;         // aggregated volume v', [y][x][Dpad]
   uint32_t* cand;        // per pixel 4 words: key0, key1, (nbm | nbp << 16), 0
+  int16_t* rdisp;        // exact reverse match per right pixel (lrc only), W*H
   float* disp;           // outputs, W*H (NaN = no answer)
   float* margin;
   float* cand6;          // nullable: s1, s2, d1, d2, lrc, lrc2 planes

@@ -551,4 +551,67 @@ hipError_t launch_census_plane(const uint8_t* img, int W, int H, int hw, int hh,
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Exact reverse match (--lrc, core/tools/de_dense.cpp:1438-1453, 1581-1597):
+// for every right pixel, the disparity index whose final filtered score is the
+// largest over all left pixels claiming it (x = xr + d), the smallest index
+// among equals; then the residual plane |d(x) - (dmin + that index at
+// x - round(d(x)))|. Reads the volume the column pass stored back.
+__global__ __launch_bounds__(256) void k_lrc_exact(BatchView b) {
+  const int item = blockIdx.x;
+  const int f = find_frame(b, kStageSolve, item);
+  const FrameDesc F = b.frames[f];
+  if (!F.lrc) return;
+  const int y = local_item(b, kStageSolve, item, f);
+  const int W = F.W, Dp = F.Dpad, D = F.D;
+  const int lane = threadIdx.x & 63;
+  const bool yok = y >= F.hh && y < F.H - F.hh;
+  for (int xr = threadIdx.x >> 6; xr < W; xr += 4) {
+    uint32_t key = 0u;
+    if (yok && xr >= F.hw) {
+      for (int k = lane; k < D; k += kWave) {
+        const int x = xr + F.dmin + k;
+        if (x >= W - F.hw) break;
+        const uint32_t v = F.vol[(size_t(y) * W + x) * Dp + k];
+        key = max(key, pack_key(v, k));
+      }
+    }
+    for (int o = 1; o < kWave; o <<= 1) key = max(key, uint32_t(__shfl_xor(int(key), o)));
+    if (lane == 0) F.rdisp[size_t(y) * W + xr] = key ? int16_t(key_k(key)) : int16_t(-1);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_lrc_plane(BatchView b) {
+  const int item = blockIdx.x;
+  const int f = find_frame(b, kStageSolve, item);
+  const FrameDesc F = b.frames[f];
+  if (!F.lrc || !F.cand6) return;
+  const int y = local_item(b, kStageSolve, item, f);
+  const int W = F.W;
+  const size_t WH = size_t(W) * F.H;
+  for (int x = threadIdx.x; x < W; x += 256) {
+    const size_t i = size_t(y) * W + x;
+    const float dl = F.disp[i];
+    float r = __int_as_float(0x7fc00000);
+    if (isfinite(dl)) {
+      const int xr = x - int(lroundf(dl));
+      if (xr >= 0 && xr < W) {
+        const int rd = F.rdisp[size_t(y) * W + xr];
+        if (rd >= 0) r = fabsf(dl - float(F.dmin + rd));
+      }
+    }
+    F.cand6[4 * WH + i] = r;
+  }
+}
+
+hipError_t launch_lrc_exact(const BatchView& b, int items, hipStream_t s) {
+  hipLaunchKernelGGL(k_lrc_exact, dim3(items), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_lrc_plane(const BatchView& b, int items, hipStream_t s) {
+  hipLaunchKernelGGL(k_lrc_plane, dim3(items), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
 }  // namespace de

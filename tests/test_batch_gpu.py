@@ -45,7 +45,7 @@ SHAPES = [(320, 240, 64), (424, 240, 96), (640, 360, 128), (200, 120, 192),
 def cfg_for(D, i):
     from doubleeye_amd.dense import Cfg
     return Cfg(dmax=D, iters=(8 if i % 3 else 0), sigma_s=20.0, sigma_r=0.0392,
-               min_margin=(0.01 if i == 5 else 0.0))
+               min_margin=(0.01 if i == 5 else 0.0), lrc=(i in (2, 6)))
 
 
 def test_batch_equals_per_pair(ctx):

@@ -41,9 +41,9 @@ def check_dense(ctx, oracle, L, R, kw, tag):
     d, m, cand = ctx.run_dense(L, R, dcfg(**kw), want_cand=True)
     od, om, oc, ties = oracle.dense_run(L, R, oracle.cfg(**kw), want_cand=True, want_ties=True)
     assert same_f32(m, om).all(), f"{tag}: margin differs at {(~same_f32(m, om)).sum()} px"
-    for p in (0, 1, 2, 3, 5):
+    for p in (0, 1, 2, 3, 4, 5):
         eq = same_f32(cand[p], oc[p])
-        if p == 5:    # lrc2 follows the disparity; only tie pixels may move it
+        if p in (4, 5):   # lrc, lrc2 follow the disparity; only tie pixels may move them
             eq |= ties.astype(bool)
         assert eq.all(), f"{tag}: cand plane {p} differs at {(~eq).sum()} px"
     eq = same_f32(d, od)
@@ -81,12 +81,25 @@ def test_c1_fixture_all_flag_sets(ctx, oracle):
     z = np.load(os.path.join(GOLDEN, "dense_c1.npz"))
     for name in ("c1", "default48", "nosub_margin", "parabola_ratio", "d96_it8", "lrc_it2"):
         kw = flags_to_kwargs(str(z[f"{name}_flags"]))
-        kw.pop("lrc", None)      # exact reverse-match plane: not on the device yet
         d, od, ties = check_dense(ctx, oracle, z["L"], z["R"], kw, name)
         if name == "c1":
             # the oracle equals the reference here (pinned by test_oracle)
             eq = same_f32(d, z["c1_disp"]) | ties.astype(bool)
             assert eq.all()
+
+
+def test_exact_reverse_match_plane(ctx, oracle):
+    """--lrc (de_dense.cpp:1438-1453, 1581-1597) on larger, slanted pairs and
+    the 9x7 window; plane 4 is -1e30 everywhere without --lrc."""
+    from doubleeye_amd import synth
+    for (W, H, D, win, it) in ((424, 240, 96, (3, 3), 4), (320, 200, 64, (4, 3), 0)):
+        L, R, _ = synth.slanted_pair(W, H, seed=12, dhi=0.7 * D)
+        kw = dict(dmax=D, iters=it, sigma_s=20.0, sigma_r=0.0392, lrc=1,
+                  census_hw=win[0], census_hh=win[1])
+        check_dense(ctx, oracle, L, R, kw, f"lrc {W}x{H} D{D} {win}")
+    kw["lrc"] = 0
+    _, _, cand = ctx.run_dense(L, R, dcfg(**kw), want_cand=True)
+    assert (cand[4] == np.float32(-1e30)).all()
 
 
 @pytest.mark.parametrize("scene", ["night_off", "morning_on", "evening_on"])
@@ -130,7 +143,7 @@ def test_invalid_arguments_raise(ctx):
     from doubleeye_amd._lib import DeError
     L = np.zeros((32, 32), np.uint8)
     for kw in (dict(dmax=0), dict(dmax=400), dict(census_hw=5), dict(sigma_s=0.0),
-               dict(iters=-1), dict(lrc=True)):
+               dict(iters=-1)):
         with pytest.raises(DeError):
             ctx.run_dense(L, L, dcfg(**kw))
     with pytest.raises(ValueError):
