@@ -169,6 +169,13 @@ SIGNATURES = {
     "de_seg_max_excluding": (I, [P, P, P, C.c_int64, C.c_int64, P]),
     "de_dense_masda": (I, [P, P, P, I, I, I, I, I, I, C.c_double, C.c_double, I, C.c_double,
                            C.c_double, P, P, C.POINTER(I)]),
+    "de_stream_packet_bytes": (C.c_size_t, [I, I, I]),
+    "de_stream_create": (I, [P, I, I, C.POINTER(DeCfgC), I, C.POINTER(P)]),
+    "de_stream_destroy": (None, [P]),
+    "de_stream_feed": (I, [P, P, C.c_size_t, P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "de_stream_stats": (I, [P, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    "de_stream_run_fd": (I, [P, I, I, I, I, C.POINTER(DeCfgC), I, C.POINTER(C.c_long),
+                             C.POINTER(C.c_long)]),
     "de_server_default_cfg": (None, [C.POINTER(DeServerCfgC)]),
     "de_server_create": (I, [P, C.POINTER(DeServerCfgC), C.POINTER(P)]),
     "de_server_destroy": (None, [P]),

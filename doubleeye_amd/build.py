@@ -34,6 +34,7 @@ SOURCES = {
     "de_serve.hip": [],
     "de_edges.hip": ["-ffp-contract=off"],
     "de_fp32.hip": ["-ffp-contract=off"],
+    "de_stream.hip": [],
 }
 
 

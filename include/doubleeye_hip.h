@@ -387,6 +387,30 @@ int de_dense_masda(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H
                    int dmax, int census_hw, int census_hh, double lam, double gam, int iters,
                    double damping, double eps, float* disp, double* hist, int* n_iter);
 
+/* ==== DEIR -> DEDD stream front end ========================================
+ * The packet grammar and pairing of `de_dense_cuda --stream`
+ * (core/tools/de_dense_cuda.cu:44-59, 670-770):
+ *   in:  "DEIR" | w u16 | h u16 | stream u8 | flags u8 | frame u32 | w*h Y8
+ *   out: "DEDD" | w u16 | h u16 | frame u32 | flags u32 | left Y8 |
+ *        w*h int16 Q4 disparity (-32768 = none) | w*h u8 confidence if flags & 1
+ * Images wait by frame number (8 at most, the oldest dropped beyond that); a
+ * second image of the same stream replaces the first; stream 1 is the left
+ * camera. A bad magic or geometry returns DE_ERR_ARG (the reference exits 1). */
+typedef struct de_stream de_stream;
+/* Output bytes of one DEDD packet. */
+size_t de_stream_packet_bytes(int W, int H, int want_conf);
+int de_stream_create(de_ctx* ctx, int W, int H, const de_cfg* cfg, int want_conf,
+                     de_stream** out);
+void de_stream_destroy(de_stream* s);
+/* Feed any number of input bytes; complete DEDD packets are written to out
+ * (capacity out_cap; de_stream_packet_bytes per pair completed). */
+int de_stream_feed(de_stream* s, const uint8_t* data, size_t n, uint8_t* out, size_t out_cap,
+                   size_t* out_len);
+int de_stream_stats(de_stream* s, long* pairs, long* dropped);
+/* The whole loop over file descriptors until end of input. */
+int de_stream_run_fd(de_ctx* ctx, int in_fd, int out_fd, int W, int H, const de_cfg* cfg,
+                     int want_conf, long* pairs, long* dropped);
+
 /* ==== Serving loop =========================================================
  * Many camera streams of mixed geometry on one GPU, with continuous batching.
  * The reference serves one fixed-geometry stream with one frame in flight

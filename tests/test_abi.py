@@ -41,3 +41,22 @@ def test_cfg_struct_layout_and_defaults():
     py = Cfg().to_c()
     for name, _ in _lib.DeCfgC._fields_:
         assert getattr(c, name) == getattr(py, name), name
+
+
+def test_deir_dedd_packet_helpers():
+    """Packet helpers against hand-built bytes (no GPU)."""
+    import struct
+
+    import numpy as np
+
+    from doubleeye_amd.stream import deir_packet, parse_dedd
+    img = np.arange(12, dtype=np.uint8).reshape(3, 4)
+    p = deir_packet(img, 1, 77)
+    assert p[:4] == b"DEIR" and struct.unpack_from("<HHBBI", p, 4) == (4, 3, 1, 0, 77)
+    assert p[14:] == img.tobytes() and len(p) == 14 + 12
+    q4 = np.array([[16, -32768, 5, 0]] * 3, np.int16)
+    conf = np.full((3, 4), 9, np.uint8)
+    pkt = b"DEDD" + struct.pack("<HHII", 4, 3, 77, 1) + img.tobytes() + q4.tobytes() + conf.tobytes()
+    (d,) = parse_dedd(pkt)
+    assert d["frame"] == 77 and np.array_equal(d["left"], img)
+    assert np.array_equal(d["q4"], q4) and np.array_equal(d["conf"], conf)
