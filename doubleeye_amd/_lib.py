@@ -108,6 +108,26 @@ class DeServerResultC(C.Structure):
                 ("knn", C.c_void_p)]
 
 
+class DeDetectorCfgC(C.Structure):
+    """de_detector_cfg (include/doubleeye_hip.h)."""
+    _fields_ = [("grad_window", C.c_int), ("cell", C.c_int), ("per_cell", C.c_int),
+                ("min_response", C.c_float), ("min_local_std", C.c_float),
+                ("border", C.c_int), ("subpixel", C.c_int), ("fast_threshold", C.c_int),
+                ("nms_radius", C.c_int)]
+
+
+class DeDetectProfileC(C.Structure):
+    """de_detect_profile (include/doubleeye_hip.h)."""
+    _fields_ = [("fast_ms", C.c_double), ("nms_ms", C.c_double), ("refine_ms", C.c_double),
+                ("candidates", C.c_int), ("suppressed", C.c_int)]
+
+
+class DeDetectFrameC(C.Structure):
+    """de_detect_frame (include/doubleeye_hip.h): device pointers of one image."""
+    _fields_ = [("img", C.c_void_p), ("W", C.c_int), ("H", C.c_int), ("out", C.c_void_p),
+                ("cap", C.c_int), ("n_out", C.c_void_p)]
+
+
 # (name, restype, argtypes) for every exported entry point
 P = C.c_void_p
 I = C.c_int
@@ -186,6 +206,18 @@ SIGNATURES = {
     "de_server_submit": (I, [P, I, P, P, P, I, C.c_uint64, C.POINTER(I)]),
     "de_server_poll": (I, [P, C.POINTER(DeServerResultC), I, I, C.POINTER(I)]),
     "de_server_release": (I, [P, I, I]),
+    "de_detector_default_cfg": (None, [C.POINTER(DeDetectorCfgC)]),
+    "de_detect_max_keypoints": (C.c_size_t, [I, I, C.POINTER(DeDetectorCfgC)]),
+    "de_detect_keypoints_fast": (I, [P, P, I, I, C.POINTER(DeDetectorCfgC), P, I, C.POINTER(I),
+                                     P]),
+    "de_detect_keypoints_fast_batch_dev": (I, [P, C.POINTER(DeDetectFrameC),
+                                               C.POINTER(DeDetectorCfgC), I, P]),
+    "de_fast_score_map": (I, [P, P, I, I, I, I, P, P]),
+    "de_census_at_keypoints": (I, [P, P, I, I, I, I, P, I, P]),
+    "de_refine_disparity": (I, [P, P, P, I, I, P, I, P, I, P, I, I]),
+    "de_sparse_pipeline": (I, [P, P, P, I, I, C.POINTER(DeDetectorCfgC), P, I, I,
+                               C.POINTER(DeMatchCfgC), F, I, P, I, C.POINTER(I), P, I,
+                               C.POINTER(I), P, P, I, C.POINTER(I), P]),
 }
 
 _lib = None

@@ -381,7 +381,8 @@ Ctx::~Ctx() {
     for (EvSet& ev : *v)
       for (int i = 0; i <= kNumStages; ++i) (void)hipEventDestroy(ev.e[i]);
   for (DevBuf* b : {&scratch, &rf_buf, &sc_buf, &io_in, &io_out, &sp_scratch, &sp_io,
-                    &kb_scratch, &kb_io, &ed_scratch, &ed_io, &f32_scratch, &f32_io})
+                    &kb_scratch, &kb_io, &ed_scratch, &ed_io, &f32_scratch, &f32_io,
+                    &det_scratch, &det_io, &pipe_io})
     b->release();
   if (stream) (void)hipStreamDestroy(stream);
 }

@@ -62,6 +62,8 @@ struct Ctx {
   DevBuf kb_scratch, kb_io;   // keypoint-graph BP: scratch, staged inputs/outputs
   DevBuf ed_scratch, ed_io;   // edge-list MASDA: scratch, staged inputs/outputs
   DevBuf f32_scratch, f32_io; // FP32 full-volume path: scratch, staged inputs/outputs
+  DevBuf det_scratch, det_io; // keypoint detector: scratch, staged images and keypoints
+  DevBuf pipe_io;             // detect -> match pipeline: descriptors and matches
   Slot slots[kSlots];
   int next_slot = 0;
   bool store_vol = false;
@@ -128,6 +130,19 @@ size_t knn_scratch_bytes(int W, int H, int n);
 int knn_validate(const de_knn_cfg& c, int W, int H, int n);
 int knn_launch_frames(Ctx& x, const de_knn_frame* frames, const de_knn_cfg* cfgs,
                       void* const* scratch, int n, hipStream_t s);
+
+// sparse MASDA of one frame whose keypoints and descriptors are on the device;
+// launches on s without synchronising
+int sp_validate_cfg(const de_match_cfg& c, int nl, int nr);
+int sparse_match_dev(Ctx& x, const de_keypoint* L, const uint64_t* dl, int nl,
+                     const de_keypoint* R, const uint64_t* dr, int nr, const de_match_cfg& cfg,
+                     de_match* out, int* n_out, de_match_stats* stats, hipStream_t s);
+
+// keypoint detector
+int det_validate(const de_detector_cfg& c, int W, int H);
+size_t det_max_keypoints(const de_detector_cfg& c, int W, int H);
+int det_launch(Ctx& x, const de_detect_frame* frames, const de_detector_cfg* cfgs, int n,
+               hipStream_t s, hipEvent_t* ev);
 
 }  // namespace de
 

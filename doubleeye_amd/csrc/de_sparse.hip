@@ -918,6 +918,30 @@ static int sp_host(de_ctx* c, const de_keypoint* left, const uint64_t* dl, int n
   return DE_OK;
 }
 
+int sp_validate_cfg(const de_match_cfg& c, int nl, int nr) { return sp_validate(c, nl, nr); }
+
+int sparse_match_dev(Ctx& x, const de_keypoint* L, const uint64_t* dl, int nl,
+                     const de_keypoint* R, const uint64_t* dr, int nr, const de_match_cfg& cfg,
+                     de_match* out, int* n_out, de_match_stats* stats, hipStream_t s) {
+  if (int rc = sp_validate(cfg, nl, nr)) return rc;
+  SpDesc d;
+  std::memset(&d, 0, sizeof(d));
+  sp_fill_cfg(cfg, kModeMasda, 0.f, &d);
+  d.L = reinterpret_cast<const float4*>(L);
+  d.dl = dl;
+  d.R = reinterpret_cast<const float4*>(R);
+  d.dr = dr;
+  d.nl = nl;
+  d.nr = nr;
+  d.out = out;
+  d.n_out = n_out;
+  d.stats = stats;
+  if (int rc = x.sp_scratch.ensure(sp_scratch_bytes(nl, nr, d.K))) return rc;
+  sp_carve(x.sp_scratch.ptr, nl, nr, d.K, &d);
+  std::vector<SpDesc> fr(1, d);
+  return sp_run(x, fr, true, s);
+}
+
 }  // namespace de
 
 using namespace de;

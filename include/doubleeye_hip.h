@@ -276,6 +276,89 @@ typedef struct de_sparse_frame {
 int de_match_masda_batch_dev(de_ctx* ctx, const de_sparse_frame* frames,
                              const de_match_cfg* cfgs, int n, void* stream);
 
+/* ==== Keypoint detector and the detect -> match pipeline ===================
+ * detect_keypoints_fast (core/src/preproc.cpp:436-545): FAST-9 candidates,
+ * Shi-Tomasi response at candidates, greedy suppression within nms_radius in
+ * descending response order, texture floor, sub-pixel parabola, the top
+ * per_cell per cell x cell block; keypoints in row-major cell order, each
+ * cell's in descending response. Equal responses are ordered by scan position
+ * (y, then x), where the reference's std::sort leaves their order open. */
+
+/* doubleeye::DetectorConfig (core/include/doubleeye/preproc.hpp:132-145);
+ * subpixel is 0 or 1. */
+typedef struct de_detector_cfg {
+  int grad_window, cell, per_cell;
+  float min_response, min_local_std;
+  int border, subpixel, fast_threshold, nms_radius;
+} de_detector_cfg;
+
+/* doubleeye::DetectProfile (preproc.hpp:191-198): stage times from GPU events. */
+typedef struct de_detect_profile {
+  double fast_ms, nms_ms, refine_ms;
+  int candidates, suppressed;
+} de_detect_profile;
+
+/* One image of a device detector batch. Device pointers; cap >=
+ * de_detect_max_keypoints(W, H, cfg); n_out is a single device int. */
+typedef struct de_detect_frame {
+  const uint8_t* img;
+  int W, H;
+  de_keypoint* out;
+  int cap;
+  int* n_out;
+} de_detect_frame;
+
+/* DetectorConfig's defaults. */
+void de_detector_default_cfg(de_detector_cfg* cfg);
+
+/* Upper bound on the keypoints of one image: min(cells * per_cell, pixels
+ * inside the border). */
+size_t de_detect_max_keypoints(int W, int H, const de_detector_cfg* cfg);
+
+/* detect_keypoints_fast (preproc.hpp:202-204). Host pointers; fails with
+ * DE_ERR_ARG when more than `cap` keypoints result (*n_out then holds the
+ * count). prof is nullable; its stage times come from GPU events.
+ * nms_radius is limited to 255. */
+int de_detect_keypoints_fast(de_ctx* ctx, const uint8_t* img, int W, int H,
+                             const de_detector_cfg* cfg, de_keypoint* out, int cap, int* n_out,
+                             de_detect_profile* prof);
+
+/* The detector over n images with per-image configurations, in one chain of
+ * launches on `stream` without host synchronisation. Results equal n separate
+ * de_detect_keypoints_fast calls. No counterpart in the reference. */
+int de_detect_keypoints_fast_batch_dev(de_ctx* ctx, const de_detect_frame* frames,
+                                       const de_detector_cfg* cfgs, int n, void* stream);
+
+/* is_fast_corner (preproc.hpp:179) and shi_tomasi_at (preproc.hpp:183) at
+ * every pixel: corner[y*W+x] in {0, 1}, response[y*W+x]. For parity tests. */
+int de_fast_score_map(de_ctx* ctx, const uint8_t* img, int W, int H, int threshold,
+                      int grad_window, uint8_t* corner, float* response);
+
+/* census_at (preproc.hpp:96-97) at (int(x + 0.5f), int(y + 0.5f)) of each
+ * keypoint, as core/tools/de_pipe.cpp:168-172 describes them. */
+int de_census_at_keypoints(de_ctx* ctx, const uint8_t* img, int W, int H, int hw, int hh,
+                           const de_keypoint* kp, int n, uint64_t* desc);
+
+/* refine_disparity (match.hpp:293-296, match.cpp:768-810): in place on the
+ * host matches; half is the SAD half-window (the reference's default is 3). */
+int de_refine_disparity(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+                        const de_keypoint* kl, int n_left, const de_keypoint* kr, int n_right,
+                        de_match* matches, int n, int half);
+
+/* One stereo pair through core/tools/de_pipe.cpp:167-199: detect on both images
+ * (det_right nullable = det_left), census_at at the keypoints, match_masda,
+ * drop matches whose margin is below min_margin (when > 0), refine_disparity
+ * (half 3) when subpixel. Outputs the keypoints, the matches and, when
+ * `records` is non-null, the DEMR records (x_left, y_left, disparity, margin)
+ * as 4 floats per match. Capacities: kl_cap / kr_cap from
+ * de_detect_max_keypoints, out_cap >= min(n_left, n_right). */
+int de_sparse_pipeline(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+                       const de_detector_cfg* det_left, const de_detector_cfg* det_right,
+                       int census_hw, int census_hh, const de_match_cfg* mcfg, float min_margin,
+                       int subpixel, de_keypoint* kl, int kl_cap, int* n_kl, de_keypoint* kr,
+                       int kr_cap, int* n_kr, de_match* out, float* records, int out_cap,
+                       int* n_out, de_match_stats* stats);
+
 /* ==== Keypoint-graph belief propagation ====================================
  * No counterpart in the reference (its sparse matcher is bipartite). For
  * keypoints of the left image: labels are the n_cand disparities along the

@@ -139,6 +139,112 @@ int ref_match_brute_force(const OrcKeypoint* L, const uint64_t* dl, int nl,
   return 0;
 }
 
+// detect_keypoints_fast (core/include/doubleeye/preproc.hpp:202-204) on a
+// W*H Y8 image. Config fields in DetectorConfig order; counts (nullable) gets
+// DetectProfile's candidates and suppressed.
+int ref_detect_keypoints_fast(const uint8_t* img, int W, int H, const int* icfg,
+                              const float* fcfg, OrcKeypoint* out, int cap, int* n_out,
+                              int* counts) {
+  Image8 im(W, H);
+  std::memcpy(im.data.data(), img, size_t(W) * H);
+  DetectorConfig c;
+  c.grad_window = icfg[0];
+  c.cell = icfg[1];
+  c.per_cell = icfg[2];
+  c.min_response = fcfg[0];
+  c.min_local_std = fcfg[1];
+  c.border = icfg[3];
+  c.subpixel = icfg[4] != 0;
+  c.fast_threshold = icfg[5];
+  c.nms_radius = icfg[6];
+  DetectProfile prof;
+  const std::vector<Keypoint> k = detect_keypoints_fast(im, c, &prof);
+  if (counts) {
+    counts[0] = prof.candidates;
+    counts[1] = prof.suppressed;
+  }
+  if (int(k.size()) > cap) return -1;
+  for (size_t i = 0; i < k.size(); ++i) {
+    out[i].x = k[i].x;
+    out[i].y = k[i].y;
+    out[i].response = k[i].response;
+    out[i].local_std = k[i].local_std;
+  }
+  *n_out = int(k.size());
+  return 0;
+}
+
+int ref_is_fast_corner(const uint8_t* img, int W, int H, int x, int y, int thr) {
+  Image8 im(W, H);
+  std::memcpy(im.data.data(), img, size_t(W) * H);
+  return is_fast_corner(im, x, y, thr) ? 1 : 0;
+}
+
+// shi_tomasi_at over every pixel (preproc.hpp:183), and the dense
+// shi_tomasi_response (preproc.hpp:151) for comparison.
+void ref_shi_tomasi_maps(const uint8_t* img, int W, int H, int grad_window, float* at,
+                         float* dense) {
+  Image8 im(W, H);
+  std::memcpy(im.data.data(), img, size_t(W) * H);
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) at[size_t(y) * W + x] = shi_tomasi_at(im, x, y, grad_window);
+  const std::vector<float> d = shi_tomasi_response(im, grad_window);
+  std::memcpy(dense, d.data(), d.size() * sizeof(float));
+}
+
+// refine_disparity (match.hpp:293-296, match.cpp:768-810) on matches in place.
+void ref_refine_disparity(const uint8_t* L, const uint8_t* R, int W, int H,
+                          const OrcKeypoint* kl, int nl, const OrcKeypoint* kr, int nr,
+                          OrcMatch* m, int n, int half) {
+  Image8 li(W, H), ri(W, H);
+  std::memcpy(li.data.data(), L, size_t(W) * H);
+  std::memcpy(ri.data.data(), R, size_t(W) * H);
+  std::vector<Keypoint> a(static_cast<size_t>(nl)), b(static_cast<size_t>(nr));
+  for (int i = 0; i < nl; ++i) a[size_t(i)] = Keypoint{kl[i].x, kl[i].y, kl[i].response, kl[i].local_std};
+  for (int i = 0; i < nr; ++i) b[size_t(i)] = Keypoint{kr[i].x, kr[i].y, kr[i].response, kr[i].local_std};
+  std::vector<Match> v(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) {
+    Match& q = v[size_t(i)];
+    q.left = m[i].left;
+    q.right = m[i].right;
+    q.score = m[i].score;
+    q.belief = m[i].belief;
+    q.disparity = m[i].disparity;
+    q.dy = m[i].dy;
+    q.margin = m[i].margin;
+  }
+  refine_disparity(li, ri, a, b, &v, half);
+  for (int i = 0; i < n; ++i) m[i].disparity = v[size_t(i)].disparity;
+}
+
+// detect_keypoints (preproc.hpp:160-161), the dense detector, for coverage
+// comparisons.
+int ref_detect_keypoints_dense(const uint8_t* img, int W, int H, const int* icfg,
+                               const float* fcfg, OrcKeypoint* out, int cap, int* n_out) {
+  Image8 im(W, H);
+  std::memcpy(im.data.data(), img, size_t(W) * H);
+  DetectorConfig c;
+  c.grad_window = icfg[0];
+  c.cell = icfg[1];
+  c.per_cell = icfg[2];
+  c.min_response = fcfg[0];
+  c.min_local_std = fcfg[1];
+  c.border = icfg[3];
+  c.subpixel = icfg[4] != 0;
+  c.fast_threshold = icfg[5];
+  c.nms_radius = icfg[6];
+  const std::vector<Keypoint> k = detect_keypoints(im, c);
+  if (int(k.size()) > cap) return -1;
+  for (size_t i = 0; i < k.size(); ++i) {
+    out[i].x = k[i].x;
+    out[i].y = k[i].y;
+    out[i].response = k[i].response;
+    out[i].local_std = k[i].local_std;
+  }
+  *n_out = int(k.size());
+  return 0;
+}
+
 float ref_matching_objective(const OrcMatch* m, int n, const OrcMatchCfg* cfg,
                              int n_left, int n_right) {
   Cfg c;

@@ -83,6 +83,26 @@ int orc_match_brute_force(const OrcKeypoint* left, const uint64_t* dl, int nl,
 float orc_matching_objective(const OrcMatch* m, int n, const OrcMatchCfg* cfg,
                              int n_left, int n_right);
 
+/* FAST keypoint detector (core/src/preproc.cpp:371-545) and refine_disparity
+ * (core/src/match.cpp:768-810); see detect_ref.cpp. DetectorConfig field order
+ * (core/include/doubleeye/preproc.hpp:132-145). */
+typedef struct OrcDetectorCfg {
+  int grad_window, cell, per_cell;
+  float min_response, min_local_std;
+  int border, subpixel, fast_threshold, nms_radius;
+} OrcDetectorCfg;
+void orc_detector_default_cfg(OrcDetectorCfg* c);
+int orc_is_fast_corner(const uint8_t* img, int W, int H, int x, int y, int thr);
+float orc_shi_tomasi_at(const uint8_t* img, int W, int H, int x, int y, int grad_window);
+float orc_local_std_at(const uint8_t* img, int W, int H, int x, int y);
+/* counts (nullable): [0] FAST candidates above min_response, [1] survivors of
+ * suppression, as DetectProfile reports them (preproc.hpp:191-198). */
+int orc_detect_keypoints_fast(const uint8_t* img, int W, int H, const OrcDetectorCfg* cfg,
+                              int canonical, OrcKeypoint* out, int cap, int* n_out, int* counts);
+void orc_refine_disparity(const uint8_t* L, const uint8_t* R, int W, int H,
+                          const OrcKeypoint* kl, int nl, const OrcKeypoint* kr, int nr,
+                          OrcMatch* m, int n, int half);
+
 #ifdef __cplusplus
 }
 #endif
