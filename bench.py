@@ -515,6 +515,145 @@ def bench_dense_masda(a):
     print(json.dumps(out), flush=True)
 
 
+def _scene_pairs():
+    import numpy as np
+    z = np.load(os.path.join(ROOT, "tests", "golden", "scenes.npz"))
+    names = ["night_on", "night_off", "evening_on", "evening_off", "morning_on", "morning_off"]
+    return names, [(z[f"{s}_L"], z[f"{s}_R"]) for s in names]
+
+
+def bench_detect(a):
+    """detect_keypoints_fast (DetectorConfig defaults) on the reference's six
+    848x480 IR scene pairs (12 images), images resident in HBM: one image per
+    launch chain, and all 12 in one batch. Per-stage times come from the host
+    entry's DetectProfile on the first image. The CPU baseline is the
+    reference's own detect_keypoints_fast (oracle/_ref/libref_sparse.so, built
+    from core/src/preproc.cpp), single thread, on the same 12 images."""
+    import numpy as np
+
+    from doubleeye_amd import detect, sparse
+    from doubleeye_amd.dense import Context
+
+    ctx = Context(0)
+    _, pairs = _scene_pairs()
+    imgs = [im for p in pairs for im in p]
+    cfg = detect.DetectorConfig()
+    cap = detect.max_keypoints(848, 480, cfg)
+    frames = [detect.DetectFrame(ctx.to_device(im), ctx.empty((cap,), sparse.KEYPOINT),
+                                 ctx.empty((1,), np.int32)) for im in imgs]
+    prof = detect.DetectProfile()
+    want = [detect.detect_keypoints_fast(im, cfg, prof if k == 0 else None, ctx=ctx)
+            for k, im in enumerate(imgs)]
+    reps = max(20, a.steps)
+    for f in frames[:1] * 5:
+        detect.detect_keypoints_fast_batch_dev(ctx, [f], [cfg])
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(reps):
+        detect.detect_keypoints_fast_batch_dev(ctx, [frames[k % 12]], [cfg])
+    ctx.sync()
+    ms_one = (time.perf_counter() - t0) * 1e3 / reps
+    detect.detect_keypoints_fast_batch_dev(ctx, frames, [cfg] * 12)
+    ctx.sync()
+    same = all(np.array_equal(f.out.numpy()[:int(f.n_out.numpy()[0])].view(np.uint32),
+                              w.view(np.uint32)) for f, w in zip(frames, want))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        detect.detect_keypoints_fast_batch_dev(ctx, frames, [cfg] * 12)
+    ctx.sync()
+    ms_batch = (time.perf_counter() - t0) * 1e3 / reps
+    out = {
+        "metric": "detect_keypoints_fast images/s (848x480 IR scenes, default config)",
+        "value": round(1e3 / ms_one, 1), "unit": "images/s",
+        "n_gpus": 1, "steps": reps, "warmup": 5, "ms_per_step": round(ms_one, 4),
+        "batch12_ms": round(ms_batch, 4), "batch12_images_per_s": round(12e3 / ms_batch, 1),
+        "keypoints_per_image": round(float(np.mean([len(w) for w in want])), 1),
+        "stage_ms_first_image": {"fast": round(prof.fast_ms, 4), "nms": round(prof.nms_ms, 4),
+                                 "refine": round(prof.refine_ms, 4)},
+        "candidates_first_image": prof.candidates, "suppressed_first_image": prof.suppressed,
+        "batch_identical_to_single_calls": same,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8 pixels / f32 response / f64 texture", "data": "the reference's IR scenes",
+        "config": {"workload": "detect_keypoints_fast, 848x480, DetectorConfig defaults"},
+    }
+    if not a.no_cpu_baseline:
+        try:
+            sys.path.insert(0, ROOT)
+            from oracle import detect_binding as db
+            c = db.detector_cfg()
+            n, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < min(a.cpu_budget, 5.0):
+                db.ref_detect_keypoints_fast(imgs[n % 12], c)
+                n += 1
+            ms = (time.perf_counter() - t0) * 1e3 / n
+            out["cpu_baseline"] = {"value": round(1e3 / ms, 1), "unit": "images/s", "cores": 1,
+                                   "kind": "reference", "ms_per_image": round(ms, 3),
+                                   "sample": f"{n} images cycling over the 12"}
+        except Exception as e:   # reference harness absent on this host
+            out["cpu_baseline"] = {"error": str(e)}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_pipe(a):
+    """core/tools/de_pipe.cpp's per-pair work (detect both images, census_at,
+    match_masda with de_pipe's settings, refine_disparity) on the six IR scene
+    pairs, host images in, DEMR records out (de_sparse_pipeline). The CPU
+    baseline is the reference's own de_pipe binary (built from its sources by
+    oracle/Makefile.ref) fed the same pairs as DEIR packets, single thread."""
+    import struct
+
+    import numpy as np
+
+    from doubleeye_amd import detect
+    from doubleeye_amd.dense import Context
+
+    ctx = Context(0)
+    names, pairs = _scene_pairs()
+    mcfg = detect.pipe_match_config()
+    for L, R in pairs:
+        detect.sparse_pipeline(L, R, cfg=mcfg, ctx=ctx)
+    reps = max(12, a.steps // 2)
+    nrec = 0
+    t0 = time.perf_counter()
+    for k in range(reps):
+        L, R = pairs[k % 6]
+        nrec += len(detect.sparse_pipeline(L, R, cfg=mcfg, ctx=ctx)[3])
+    ms_ = (time.perf_counter() - t0) * 1e3 / reps
+    out = {
+        "metric": "de_pipe sparse stereo pairs/s (848x480 IR scenes, host to host)",
+        "value": round(1e3 / ms_, 1), "unit": "pairs/s",
+        "n_gpus": 1, "steps": reps, "warmup": 6, "ms_per_step": round(ms_, 4),
+        "matches_per_pair": round(nrec / reps, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8 pixels / f32 messages", "data": "the reference's IR scenes",
+        "config": {"workload": "detect x2 + census_at + match_masda (de_pipe settings) + "
+                               "refine_disparity, 848x480"},
+    }
+    if not a.no_cpu_baseline:
+        exe = os.path.join(ROOT, "oracle", "_ref", "de_pipe")
+        if os.path.exists(exe):
+            k = 0
+            chunks = []
+            for rep in range(8):
+                for L, R in pairs:
+                    for img, st in ((L, 1), (R, 0)):
+                        chunks.append(b"DEIR" + struct.pack("<HHBBI", 848, 480, st, 0, k)
+                                      + np.ascontiguousarray(img).tobytes())
+                    k += 1
+            data = b"".join(chunks)
+            t0 = time.perf_counter()
+            subprocess.run([exe], input=data, capture_output=True, check=True)
+            t = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(k / t, 1), "unit": "pairs/s", "cores": 1,
+                                   "kind": "reference", "ms_per_pair": round(t * 1e3 / k, 3),
+                                   "sample": f"{k} pairs (the six scenes x 8) through de_pipe"}
+        else:
+            out["cpu_baseline"] = {"error": "oracle/_ref/de_pipe not built on this host"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -523,10 +662,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--workload", default="c2",
-                    choices=["c2", "c3", "c4", "c5", "sparse", "dense_masda"],
+                    choices=["c2", "c3", "c4", "c5", "sparse", "dense_masda", "detect",
+                             "pipe"],
                     help="c2: one 848x480x128 pair (default, the headline); c4: 64-pair "
                          "batch; c3: keypoint-graph BP; c5: serving loop; sparse: keypoint "
-                         "match_masda")
+                         "match_masda; detect: keypoint detector; pipe: de_pipe's pair")
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one plain device-to-device copy of MIB MiB (the "
@@ -551,6 +691,12 @@ def main():
         return
     if a.workload == "dense_masda":
         bench_dense_masda(a)
+        return
+    if a.workload == "detect":
+        bench_detect(a)
+        return
+    if a.workload == "pipe":
+        bench_pipe(a)
         return
 
     import numpy as np

@@ -39,6 +39,9 @@ SOURCES = {
 }
 
 
+COMPILE_TIMEOUT_S = 900   # per source file; hipcc takes seconds on each
+
+
 def hipcc() -> str:
     for p in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if p and os.path.exists(p):
@@ -61,7 +64,13 @@ def build(verbose: bool = False) -> str:
                                             stderr=subprocess.STDOUT, text=True)))
     failed = []
     for src, p in procs:
-        out, _ = p.communicate()
+        try:
+            out, _ = p.communicate(timeout=COMPILE_TIMEOUT_S)
+        except subprocess.TimeoutExpired:
+            p.kill()
+            out, _ = p.communicate()
+            out += f"\nhipcc exceeded {COMPILE_TIMEOUT_S} s on {src}"
+            p.returncode = -1
         # hipcc's host pass does not know the gfx feature flag: harmless
         lines = [ln for ln in out.splitlines() if "packed-fp32-ops" not in ln
                  and "is not a recognized feature" not in ln]

@@ -4,16 +4,19 @@
 // detect_keypoints_fast (core/src/preproc.cpp:436-545) runs as a fixed chain of
 // launches with no host synchronisation, batched over frames (blockIdx.z /
 // blockIdx.y = frame):
-//   k_det_init     zero each frame's counters and cell counts
-//   k_det_fast     FAST-9 (preproc.cpp:371-399) from a 70x22 LDS tile per
-//                  64x16 pixels, Shi-Tomasi (preproc.cpp:401-427) at the
+//   k_det_init     zero each frame's counters and cell counts (the FAST kernel
+//                  does it when the suppression is tiled)
+//   k_det_fast     FAST-9 (preproc.cpp:371-399) from a 70x10 LDS tile per
+//                  64x4 pixels, Shi-Tomasi (preproc.cpp:401-427) at the
 //                  corners; survivors of min_response join a candidate list
-//   k_det_nms      one round of the greedy suppression, repeated kNmsRounds
+//   k_det_nms_tile suppression passes over 64x16 tiles staged in LDS with a
+//                  halo of nms_radius, each iterated to a fixed point; repeated
+//                  kNmsTilePasses (k_det_nms: global rounds for wider radii)
 //   k_det_nms_fin  one workgroup per frame finishes any undecided candidates
-//   k_det_refine   texture floor (local_std_at, 7x7, double), sub-pixel
-//                  parabola, cell counts
+//   k_det_refine   per kept candidate: texture floor (local_std_at, 7x7),
+//                  sub-pixel parabola, cell counts
 //   k_det_scan     exclusive scans over the cells
-//   k_det_scatter  candidates into per-cell lists
+//   k_det_scatter  kept candidates into per-cell lists
 //   k_det_select   per cell, the top per_cell by (response desc, scan order)
 //
 // The reference suppresses greedily: candidates in descending response order,
@@ -41,12 +44,16 @@ namespace de {
 
 namespace {
 
-constexpr int kTileW = 64, kTileH = 16, kHalo = 3;
+constexpr int kTileW = 64, kTileH = 4, kHalo = 3;      // FAST tile (one pixel per thread)
 constexpr int kLdsW = kTileW + 2 * kHalo, kLdsH = kTileH + 2 * kHalo;
 constexpr int kDetThreads = 256;
 constexpr int kFinThreads = 1024;
-constexpr int kNmsRounds = 8;       // grid-wide rounds before the finisher
-constexpr int kFlags = kNmsRounds + 1;   // [r]: some candidate undecided before round r
+constexpr int kFinList = 8192;      // undecided candidates the finisher holds in LDS
+constexpr int kNtW = 64, kNtH = 16, kNtMaxRad = 8;      // suppression tile, widest LDS halo
+constexpr int kNtLW = kNtW + 2 * kNtMaxRad, kNtLH = kNtH + 2 * kNtMaxRad;
+constexpr int kNmsTilePasses = 3;   // LDS-tile passes before the finisher
+constexpr int kNmsRounds = 8;       // global rounds before the finisher (radius > kNtMaxRad)
+constexpr int kFlags = kNmsRounds + 1;   // [r]: some candidate undecided before pass r
 constexpr int kMaxNmsRadius = 255;
 
 // FAST ring: Bresenham circle of radius 3 from the top, clockwise.
@@ -58,18 +65,22 @@ struct DetDesc {
   int W, H;
   int border, thr, r, rad;   // border = max(cfg.border, 5), r = max(1, grad_window/2)
   int cell, cols, rows, keep, subpixel, ncells;
+  int tiled;                 // 1: LDS-tile suppression (nms radius <= kNtMaxRad)
   float min_resp, min_std;
   // scratch
   uint8_t* state;            // W*H: 0 none / removed, 1 undecided, 2 kept
   float* resp;               // W*H: Shi-Tomasi response of candidates
-  uint32_t* cand;            // candidate pixel indices
-  int* cnt;                  // [0] candidates, [1] kept by suppression, [2] keypoints
-  float4* rec;               // per candidate: x, y, response, local_std
-  int* slot;                 // per candidate: place in its cell's list, -1 = dropped
+  uint32_t* cand;            // candidates left to the finisher (tiled), or all candidates
+  uint32_t* kept;            // pixel indices of the candidates suppression keeps
+  int* cnt;                  // [0] candidates, [1] kept by suppression, [2] keypoints,
+                             // [3] candidates left undecided by the tile passes
+  float4* rec;               // per kept candidate: x, y, response, local_std
+  int* slot;                 // per kept candidate: place in its cell's list, -1 = dropped
   int* cell_cnt;             // [ncells]
   int* cell_off;             // [ncells + 1]
   int* out_off;              // [ncells + 1]
-  int* items;                // candidates grouped by cell
+  int* items;                // kept candidates grouped by cell
+  int* tpend;                // [2][tiles]: tile still holds undecided candidates
   // outputs
   float4* out;
   int* n_out;
@@ -176,6 +187,13 @@ __device__ float local_std7(const uint8_t* img, int W, int H, int x, int y) {
   return var > 0.0 ? float(sqrt(var)) : 0.f;
 }
 
+// Order-preserving map of a float to uint32 (-0 and +0 map alike, as `<` sees them).
+__device__ __forceinline__ uint32_t ordered_f32(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u << 1) == 0u) u = 0u;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
 __device__ __forceinline__ uint64_t lanes_below() {
   const int lane = __lane_id();
   return lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -185,21 +203,32 @@ __device__ __forceinline__ uint64_t lanes_below() {
 
 __global__ __launch_bounds__(kDetThreads) void k_det_init(const DetDesc* __restrict__ frames,
                                                           int* __restrict__ flags) {
-  const DetDesc& d = frames[blockIdx.y];
+  const DetDesc d = frames[blockIdx.y];
   const int t = blockIdx.x * kDetThreads + threadIdx.x;
   if (t < 4) d.cnt[t] = 0;
   if (t < d.ncells) d.cell_cnt[t] = 0;
   if (blockIdx.y == 0 && t < kFlags) flags[t] = (t == 0) ? 1 : 0;
 }
 
-// FAST + Shi-Tomasi over a 64x16 block of pixels; every pixel of the image gets
-// its state written.
-__global__ __launch_bounds__(kDetThreads) void k_det_fast(const DetDesc* __restrict__ frames) {
-  const DetDesc& d = frames[blockIdx.z];
+// FAST + Shi-Tomasi over a 64x4 block of pixels, one per thread; every pixel
+// of the image gets its state written. With tiled suppression the tile passes
+// gather candidates from the state map, and block (0, 0) zeroes the frame's
+// counters (and frame 0's the pass flags): nothing else touches them before the
+// first tile pass. Otherwise candidates join a list, one atomic per block.
+__global__ __launch_bounds__(kDetThreads) void k_det_fast(const DetDesc* __restrict__ frames,
+                                                          int* __restrict__ flags) {
+  const DetDesc d = frames[blockIdx.z];
   const int W = d.W, H = d.H;
   const int x0 = blockIdx.x * kTileW, y0 = blockIdx.y * kTileH;
   if (x0 >= W || y0 >= H) return;   // uniform over the block
   __shared__ uint8_t tile[kLdsH][kLdsW];
+  __shared__ int wave_n[kDetThreads / kWave];
+  __shared__ int base_sh;
+  if (d.tiled && blockIdx.x == 0 && blockIdx.y == 0) {
+    for (int t = threadIdx.x; t < d.ncells; t += kDetThreads) d.cell_cnt[t] = 0;
+    if (threadIdx.x < 4) d.cnt[threadIdx.x] = 0;
+    if (blockIdx.z == 0 && threadIdx.x < kFlags) flags[threadIdx.x] = threadIdx.x == 0 ? 1 : 0;
+  }
   for (int i = threadIdx.x; i < kLdsH * kLdsW; i += kDetThreads) {
     const int ty = i / kLdsW, tx = i - ty * kLdsW;
     const int gx = min(max(x0 - kHalo + tx, 0), W - 1);
@@ -207,41 +236,83 @@ __global__ __launch_bounds__(kDetThreads) void k_det_fast(const DetDesc* __restr
     tile[ty][tx] = d.img[size_t(gy) * W + gx];
   }
   __syncthreads();
-  const int lx = threadIdx.x & (kWave - 1), ly0 = threadIdx.x / kWave;
+  const int lx = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   const int b = d.border, r = d.r;
-  const bool r_in_tile = r + 1 <= kHalo;
-  for (int k = 0; k < kTileH / (kDetThreads / kWave); ++k) {
-    const int ly = ly0 + k * (kDetThreads / kWave);
-    const int x = x0 + lx, y = y0 + ly;
-    bool is_cand = false;
-    if (x < W && y < H) {
-      if (x >= b && x < W - b && y >= b && y < H - b) {
-        const TilePx tp{tile, lx + kHalo, ly + kHalo};
-        if (fast_at(tp, d.thr)) {
-          float resp;
-          if (r_in_tile) resp = shi_at(tp, r);
-          else resp = shi_global(d.img, W, H, x, y, r);
-          if (!(resp < d.min_resp)) {
-            is_cand = true;
-            d.resp[size_t(y) * W + x] = resp;
-          }
+  const int x = x0 + lx, y = y0 + wv;
+  bool is_cand = false;
+  if (x < W && y < H) {
+    if (x >= b && x < W - b && y >= b && y < H - b) {
+      const TilePx tp{tile, lx + kHalo, wv + kHalo};
+      if (fast_at(tp, d.thr)) {
+        float resp;
+        if (r + 1 <= kHalo) resp = shi_at(tp, r);
+        else resp = shi_global(d.img, W, H, x, y, r);
+        if (!(resp < d.min_resp)) {
+          is_cand = true;
+          d.resp[size_t(y) * W + x] = resp;
         }
       }
-      d.state[size_t(y) * W + x] = is_cand ? 1 : 0;
     }
-    const uint64_t m = __ballot(is_cand);
-    if (m) {
-      int base = 0;
-      if (__lane_id() == 0) base = atomicAdd(&d.cnt[0], __popcll(m));
-      base = __shfl(base, 0);
-      if (is_cand) d.cand[base + __popcll(m & lanes_below())] = uint32_t(y * W + x);
-    }
+    d.state[size_t(y) * W + x] = is_cand ? 1 : 0;
+  }
+  if (d.tiled) return;   // uniform over the block
+  const uint64_t m = __ballot(is_cand);
+  if (lx == 0) wave_n[wv] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < kDetThreads / kWave; ++w) tot += wave_n[w];
+    base_sh = tot ? atomicAdd(&d.cnt[0], tot) : 0;
+  }
+  __syncthreads();
+  if (is_cand) {
+    int before = base_sh;
+    for (int w = 0; w < wv; ++w) before += wave_n[w];
+    d.cand[before + __popcll(m & lanes_below())] = uint32_t(y * W + x);
   }
 }
 
-// Decide candidate p once: 2 = kept, 0 = removed, 1 = still undecided.
+// Neighbour (dy, dx) outranks a candidate of response k: a higher response, or
+// an equal one earlier in scan order.
+__device__ __forceinline__ bool outranks(float kn, float k, int dy, int dx) {
+  return kn > k || (kn == k && (dy < 0 || (dy == 0 && dx < 0)));
+}
+
+// Decide candidate p once from global memory: 2 = kept, 0 = removed, 1 = still
+// undecided. Radii up to 3 load the whole window at once.
 __device__ __forceinline__ int nms_decide(const DetDesc& d, uint32_t p) {
   const int W = d.W, H = d.H, rad = d.rad;
+  if (rad <= 3) {
+    const int y = int(p / uint32_t(W)), x = int(p - uint32_t(y) * uint32_t(W));
+    uint8_t s[7][7];
+#pragma unroll
+    for (int a = 0; a < 7; ++a)
+#pragma unroll
+      for (int b = 0; b < 7; ++b) {
+        const int dy = a - 3, dx = b - 3, py = y + dy, px = x + dx;
+        const bool ok = (dx | dy) != 0 && abs(dy) <= rad && abs(dx) <= rad && py >= 0 &&
+                        py < H && px >= 0 && px < W;
+        s[a][b] = ok ? static_cast<const volatile uint8_t*>(d.state)[size_t(py) * W + px] : 0;
+      }
+    bool any1 = false;
+#pragma unroll
+    for (int a = 0; a < 7; ++a)
+#pragma unroll
+      for (int b = 0; b < 7; ++b) {
+        if (s[a][b] == 2) return 0;
+        any1 = any1 || s[a][b] == 1;
+      }
+    if (!any1) return 2;
+    const float k = d.resp[p];
+    bool wait = false;
+#pragma unroll
+    for (int a = 0; a < 7; ++a)
+#pragma unroll
+      for (int b = 0; b < 7; ++b)
+        if (s[a][b] == 1 && outranks(d.resp[size_t(y + a - 3) * W + (x + b - 3)], k, a - 3, b - 3))
+          wait = true;
+    return wait ? 1 : 2;
+  }
   const int y = int(p / uint32_t(W)), x = int(p - uint32_t(y) * uint32_t(W));
   const float k = d.resp[p];
   bool wait = false;
@@ -254,50 +325,305 @@ __device__ __forceinline__ int nms_decide(const DetDesc& d, uint32_t p) {
       if (px < 0 || px >= W || (dx == 0 && dy == 0)) continue;
       const uint8_t s = srow[px];
       if (s == 2) return 0;
-      if (s == 1 && !wait) {
-        const float kn = d.resp[size_t(py) * W + px];
-        if (kn > k || (kn == k && (dy < 0 || (dy == 0 && dx < 0)))) wait = true;
-      }
+      if (s == 1 && !wait && outranks(d.resp[size_t(py) * W + px], k, dy, dx)) wait = true;
     }
   }
   return wait ? 1 : 2;
 }
 
+// Append n pixel indices held in LDS to the frame's kept list (one atomic).
+__device__ __forceinline__ void append_kept(const DetDesc& d, const uint32_t* newk, int n,
+                                            int* base_sh) {
+  if (threadIdx.x == 0 && n > 0) *base_sh = atomicAdd(&d.cnt[1], n);
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) d.kept[*base_sh + j] = newk[j];
+}
+
+// One suppression pass over a 64x16 tile: the tile's undecided candidates and a
+// halo of `rad` pixels are staged in LDS. Only neighbours that outrank a
+// candidate can decide it (a kept neighbour always outranks: it was decided
+// while this candidate was undecided). For radii up to 3 the candidates are
+// ranked by priority and swept in that order, 64 at a time: each lane reads its
+// window once (in-tile neighbours compare by rank, halo ones by response), and
+// ballot rounds resolve the chunk (kept / removed / stuck behind an undecided
+// neighbour outside the tile), which reaches the fixed point of the parallel
+// rule in one sweep. Wider radii iterate the rule in LDS instead. Decided
+// states go back to global memory, newly kept candidates to the kept list;
+// pass 0 counts the candidates, the last pass hands the undecided ones to the
+// finisher.
+__global__ __launch_bounds__(kDetThreads) void k_det_nms_tile(const DetDesc* __restrict__ frames,
+                                                              int* __restrict__ flags, int pass,
+                                                              int last) {
+  if (flags[pass] == 0) return;
+  const DetDesc d = frames[blockIdx.z];
+  const int W = d.W, H = d.H, rad = d.rad;
+  const int x0 = blockIdx.x * kNtW, y0 = blockIdx.y * kNtH;
+  // uniform over the block; the host launches this kernel only for rad <= kNtMaxRad
+  if (x0 >= W || y0 >= H || rad > kNtMaxRad) return;
+  // tiles that ended the previous pass with nothing undecided stay done
+  const int tile = blockIdx.y * ((W + kNtW - 1) / kNtW) + blockIdx.x;
+  const int nt = ((W + kNtW - 1) / kNtW) * ((H + kNtH - 1) / kNtH);
+  if (pass > 0 && d.tpend[((pass - 1) & 1) * nt + tile] == 0) {
+    if (threadIdx.x == 0) d.tpend[(pass & 1) * nt + tile] = 0;
+    return;
+  }
+  constexpr int kN = kNtW * kNtH;
+  __shared__ uint8_t st[kNtLH * kNtLW];
+  __shared__ float rs[kNtLH * kNtLW];
+  __shared__ uint16_t rank[kNtLH * kNtLW];
+  __shared__ uint16_t list[kN];
+  __shared__ uint64_t key[kN];
+  __shared__ uint16_t ord[kN];
+  __shared__ uint32_t newk[kN];
+  __shared__ int nlist, nnew, npend, changed, kbase;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    nlist = 0;
+    nnew = 0;
+    npend = 0;
+  }
+  const int LW = kNtW + 2 * rad, LH = kNtH + 2 * rad;
+  const bool sweep = rad <= 3;
+  __syncthreads();
+  // stage the region: all state loads in flight together, then the responses
+  constexpr int kPer = (kNtLW * kNtLH + kDetThreads - 1) / kDetThreads;
+  uint8_t sv[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int i = t + u * kDetThreads;
+    const int ly = i / LW, lx = i - ly * LW;
+    const int gx = x0 - rad + lx, gy = y0 - rad + ly;
+    sv[u] = (i < LW * LH && gx >= 0 && gx < W && gy >= 0 && gy < H)
+                ? d.state[size_t(gy) * W + gx] : uint8_t(0);
+  }
+  float rv[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int i = t + u * kDetThreads;
+    const int ly = i / LW, lx = i - ly * LW;
+    rv[u] = sv[u] == 1 ? d.resp[size_t(y0 - rad + ly) * W + (x0 - rad + lx)] : 0.f;
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int i = t + u * kDetThreads;
+    if (i >= LW * LH) break;
+    const int ly = i / LW, lx = i - ly * LW;
+    st[i] = sv[u];
+    rs[i] = rv[u];
+    rank[i] = 0xFFFF;
+    if (sv[u] == 1 && lx >= rad && lx < rad + kNtW && ly >= rad && ly < rad + kNtH)
+      list[atomicAdd(&nlist, 1)] = uint16_t(i);
+  }
+  __syncthreads();
+  const int n = nlist;
+  if (pass == 0 && t == 0 && n > 0) atomicAdd(&d.cnt[0], n);
+  auto global_of = [&](int i) {
+    const int ly = i / LW, lx = i - ly * LW;
+    return uint32_t((y0 - rad + ly) * W + (x0 - rad + lx));
+  };
+  if (sweep && n > 0) {
+    for (int j = t; j < n; j += kDetThreads) {
+      // priority key: response, then earlier scan position inside the tile
+      // (row-major, as the image's), then j in the low 10 bits
+      const int i = list[j];
+      const int ly = i / LW, lx = i - ly * LW;
+      const int pos = (ly - rad) * kNtW + (lx - rad);
+      key[j] = (uint64_t(ordered_f32(rs[i])) << 32) | (uint64_t(kN - 1 - pos) << 10) |
+               uint64_t(j);
+    }
+    __syncthreads();
+    // rank by counting larger keys (keys are distinct), then invert
+    for (int j = t; j < n; j += kDetThreads) {
+      const uint64_t kj = key[j];
+      int r = 0, q = 0;
+      for (; q + 8 <= n; q += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) r += key[q + u] > kj ? 1 : 0;
+      }
+      for (; q < n; ++q) r += key[q] > kj ? 1 : 0;
+      rank[list[j]] = uint16_t(r);
+      ord[r] = uint16_t(j);
+    }
+    __syncthreads();
+    // sweep in rank order, 64 candidates at a time; the window of each is read
+    // once: kept neighbours remove it, in-tile neighbours of higher rank in this
+    // chunk become its lane mask, undecided ones of earlier chunks or of the halo
+    // (compared by response) leave it stuck
+    const int wv = t / kWave, lane = __lane_id();
+    for (int c0 = 0; c0 < n; c0 += kWave) {
+      if (wv == 0) {
+        const int k = c0 + lane;
+        const bool act = k < n;
+        uint64_t nb = 0;
+        bool rf = false, sf = false;
+        int i = 0;
+        if (act) {
+          i = list[ord[k]];
+          const float rk = rs[i];
+          for (int dy = -rad; dy <= rad; ++dy) {
+            const int row = i + dy * LW;
+#pragma unroll
+            for (int b = 0; b < 7; ++b) {
+              const int dx = b - 3;
+              const bool in = abs(dx) <= rad && (dx | dy) != 0;
+              const int q = in ? row + dx : i;
+              const uint8_t sq = st[q];
+              const int rq = rank[q];
+              const float fq = rs[q];
+              if (!in || sq == 0) continue;
+              if (sq == 2) {
+                rf = true;
+              } else if (rq != 0xFFFF) {
+                if (rq < k) {
+                  if (rq >= c0) nb |= 1ull << (rq - c0);
+                  else sf = true;
+                }
+              } else if (outranks(fq, rk, dy, dx)) {
+                sf = true;
+              }
+            }
+          }
+        }
+        // resolve the chunk: a lane decides once its in-chunk outranking
+        // neighbours have; the highest undecided lane always can
+        uint64_t und = __ballot(act), kept = 0, stuck = 0;
+        while (und) {
+          const bool mine = act && ((und >> lane) & 1ull) && (nb & und) == 0;
+          const bool rm = rf || (nb & kept) != 0;
+          const bool sk = !rm && (sf || (nb & stuck) != 0);
+          const uint64_t dec = __ballot(mine);
+          kept |= __ballot(mine && !rm && !sk);
+          stuck |= __ballot(mine && sk);
+          und &= ~dec;
+        }
+        if (act) {
+          const bool kp = (kept >> lane) & 1ull, sk = (stuck >> lane) & 1ull;
+          st[i] = kp ? 2 : (sk ? 1 : 0);
+          if (kp) newk[atomicAdd(&nnew, 1)] = global_of(i);
+        }
+      }
+      __syncthreads();
+    }
+  } else if (n > 0) {
+    for (;;) {
+      if (t == 0) changed = 0;
+      __syncthreads();
+      for (int j = t; j < n; j += kDetThreads) {
+        const int i = list[j];
+        if (st[i] != 1) continue;
+        int dec = 2;
+        const float k = rs[i];
+        const int ly = i / LW, lx = i - ly * LW;
+        for (int dy = -rad; dy <= rad && dec != 0; ++dy) {
+          const volatile uint8_t* srow = st + (ly + dy) * LW + lx;
+          const float* rrow = rs + (ly + dy) * LW + lx;
+          for (int dx = -rad; dx <= rad; ++dx) {
+            if (dx == 0 && dy == 0) continue;
+            const uint8_t s = srow[dx];
+            if (s == 2) {
+              dec = 0;
+              break;
+            }
+            if (s == 1 && dec == 2 && outranks(rrow[dx], k, dy, dx)) dec = 1;
+          }
+        }
+        if (dec != 1) {
+          st[i] = uint8_t(dec);
+          changed = 1;
+          if (dec == 2) newk[atomicAdd(&nnew, 1)] = global_of(i);
+        }
+      }
+      __syncthreads();
+      const int c = changed;
+      __syncthreads();
+      if (!c) break;
+    }
+  }
+  bool pend = false;
+  for (int j = t; j < n; j += kDetThreads) {
+    const int i = list[j];
+    const uint8_t s = st[i];
+    const uint32_t g = global_of(i);
+    if (s == 1) {
+      pend = true;
+      if (last) newk[kN - 1 - atomicAdd(&npend, 1)] = g;   // from the top
+    } else {
+      d.state[g] = s;
+    }
+  }
+  if (pend) flags[pass + 1] = 1;
+  const int any_pend = __syncthreads_or(pend);
+  if (t == 0) d.tpend[(pass & 1) * nt + tile] = any_pend;
+  append_kept(d, newk, nnew, &kbase);
+  if (last) {
+    __syncthreads();
+    const int np = npend;
+    if (t == 0 && np > 0) kbase = atomicAdd(&d.cnt[3], np);
+    __syncthreads();
+    for (int j = t; j < np; j += kDetThreads) d.cand[kbase + j] = newk[kN - 1 - j];
+  }
+}
+
+// One suppression round over the candidate list from global memory (radii too
+// wide for the LDS tile).
 __global__ __launch_bounds__(kDetThreads) void k_det_nms(const DetDesc* __restrict__ frames,
                                                          int* __restrict__ flags, int round) {
   if (flags[round] == 0) return;
-  const DetDesc& d = frames[blockIdx.y];
+  const DetDesc d = frames[blockIdx.y];
   const int n = d.cnt[0];
   bool pending = false;
   for (int i = blockIdx.x * kDetThreads + threadIdx.x; i < n; i += gridDim.x * kDetThreads) {
     const uint32_t p = d.cand[i];
     if (d.state[p] != 1) continue;
     const int s = nms_decide(d, p);
-    if (s == 1) pending = true;
-    else d.state[p] = uint8_t(s);
+    if (s == 1) {
+      pending = true;
+    } else {
+      d.state[p] = uint8_t(s);
+      if (s == 2) d.kept[atomicAdd(&d.cnt[1], 1)] = p;
+    }
   }
   if (pending) flags[round + 1] = 1;
 }
 
-// Rounds inside one workgroup per frame until every candidate is decided. Each
-// pass decides at least the undecided candidate of highest priority.
+// Passes inside one workgroup per frame until every candidate is decided. Each
+// pass decides at least the undecided candidate of highest priority. The
+// undecided candidates are gathered into LDS first (kFinList of them; beyond
+// that the passes walk the whole candidate list).
 __global__ __launch_bounds__(kFinThreads) void k_det_nms_fin(const DetDesc* __restrict__ frames,
                                                              const int* __restrict__ flags,
                                                              int round) {
   if (flags[round] == 0) return;
-  const DetDesc& d = frames[blockIdx.x];
-  const int n = d.cnt[0];
-  __shared__ int pending;
+  const DetDesc d = frames[blockIdx.x];
+  const int n = d.tiled ? d.cnt[3] : d.cnt[0];
+  __shared__ uint32_t open[kFinList];
+  __shared__ int n_open, pending;
+  if (threadIdx.x == 0) n_open = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kFinThreads) {
+    const uint32_t p = d.cand[i];
+    if (d.state[p] == 1) {
+      const int k = atomicAdd(&n_open, 1);
+      if (k < kFinList) open[k] = p;
+    }
+  }
+  __syncthreads();
+  const bool in_lds = n_open <= kFinList;
+  const int m = in_lds ? n_open : n;
   for (;;) {
     if (threadIdx.x == 0) pending = 0;
     __syncthreads();
     bool mine = false;
-    for (int i = threadIdx.x; i < n; i += kFinThreads) {
-      const uint32_t p = d.cand[i];
+    for (int i = threadIdx.x; i < m; i += kFinThreads) {
+      const uint32_t p = in_lds ? open[i] : d.cand[i];
       if (d.state[p] != 1) continue;
       const int s = nms_decide(d, p);
-      if (s == 1) mine = true;
-      else d.state[p] = uint8_t(s);
+      if (s == 1) {
+        mine = true;
+      } else {
+        d.state[p] = uint8_t(s);
+        if (s == 2) d.kept[atomicAdd(&d.cnt[1], 1)] = p;
+      }
     }
     if (mine) pending = 1;
     __threadfence();   // this pass's decisions reach L2 before the next pass reads them
@@ -308,57 +634,112 @@ __global__ __launch_bounds__(kFinThreads) void k_det_nms_fin(const DetDesc* __re
   }
 }
 
-__global__ __launch_bounds__(kDetThreads) void k_det_refine(const DetDesc* __restrict__ frames) {
-  const DetDesc& d = frames[blockIdx.y];
-  const int n = d.cnt[0];
-  const int W = d.W, H = d.H, r = d.r;
-  const int n_up = (n + kWave - 1) / kWave * kWave;   // whole waves for the ballot
-  for (int i = blockIdx.x * kDetThreads + threadIdx.x; i < n_up; i += gridDim.x * kDetThreads) {
-    bool kept = false;
-    uint32_t p = 0;
-    if (i < n) {
-      p = d.cand[i];
-      kept = d.state[p] == 2;
+// Texture floor and sub-pixel refinement of a kept candidate at (x, y) with
+// grad_window radius 1, from its 7x7 neighbourhood held in registers (inside
+// the image: candidates lie at least 5 pixels from every edge). The 7x7 sums
+// are integers, so int arithmetic gives the reference's double sums exactly.
+__device__ __forceinline__ float refine_r1(const uint8_t* img, int W, int x, int y, float c,
+                                           int subpixel, float min_std, float* kx, float* ky) {
+  int v[7][7];
+#pragma unroll
+  for (int a = 0; a < 7; ++a)
+#pragma unroll
+    for (int b = 0; b < 7; ++b) v[a][b] = img[size_t(y - 3 + a) * W + (x - 3 + b)];
+  int s1 = 0, s2 = 0;
+#pragma unroll
+  for (int a = 0; a < 7; ++a)
+#pragma unroll
+    for (int b = 0; b < 7; ++b) {
+      s1 += v[a][b];
+      s2 += v[a][b] * v[a][b];
     }
-    const uint64_t m = __ballot(kept);
-    if (m && __lane_id() == 0) atomicAdd(&d.cnt[1], __popcll(m));
-    if (i >= n) continue;
-    int s = -1;
-    if (kept) {
-      const int y = int(p / uint32_t(W)), x = int(p - uint32_t(y) * uint32_t(W));
-      const float ls = local_std7(d.img, W, H, x, y);
-      if (!(ls < d.min_std)) {
-        const float c = d.resp[p];
-        float kx = float(x), ky = float(y);
-        if (d.subpixel) {
-          const float l = shi_global(d.img, W, H, x - 1, y, r);
-          const float rr = shi_global(d.img, W, H, x + 1, y, r);
-          const float dxx = l - 2.f * c + rr;
-          if (fabsf(dxx) > 1e-12f) {
-            const float o = 0.5f * (l - rr) / dxx;
-            if (fabsf(o) <= 0.5f) kx += o;
-          }
-          const float u = shi_global(d.img, W, H, x, y - 1, r);
-          const float dn = shi_global(d.img, W, H, x, y + 1, r);
-          const float dyy = u - 2.f * c + dn;
-          if (fabsf(dyy) > 1e-12f) {
-            const float o = 0.5f * (u - dn) / dyy;
-            if (fabsf(o) <= 0.5f) ky += o;
-          }
+  const double mean = double(s1) / 49.0;
+  const double var = double(s2) / 49.0 - mean * mean;
+  const float ls = var > 0.0 ? float(sqrt(var)) : 0.f;
+  if (ls < min_std || !subpixel) return ls;
+  // shi_tomasi_at(x + ox, y + oy, 3) from the registers
+  auto shi = [&](int ox, int oy) {
+    auto px = [&](int dx, int dy) { return v[3 + oy + dy][3 + ox + dx]; };
+    float sxx = 0.f, syy = 0.f, sxy = 0.f;
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const float gx = (float(px(dx + 1, dy - 1)) - float(px(dx - 1, dy - 1))
+                          + 2.f * (float(px(dx + 1, dy)) - float(px(dx - 1, dy)))
+                          + float(px(dx + 1, dy + 1)) - float(px(dx - 1, dy + 1))) * 0.125f;
+        const float gy = (float(px(dx - 1, dy + 1)) - float(px(dx - 1, dy - 1))
+                          + 2.f * (float(px(dx, dy + 1)) - float(px(dx, dy - 1)))
+                          + float(px(dx + 1, dy + 1)) - float(px(dx + 1, dy - 1))) * 0.125f;
+        sxx += gx * gx;
+        syy += gy * gy;
+        sxy += gx * gy;
+      }
+    const float n = 1.f / 9.f;
+    const float a = sxx * n, b = syy * n, cc = sxy * n;
+    const float half_tr = 0.5f * (a + b);
+    const float q = 0.25f * (a - b) * (a - b) + cc * cc;
+    return half_tr - sqrtf((0.f < q) ? q : 0.f);
+  };
+  const float l = shi(-1, 0), rr = shi(1, 0);
+  const float dxx = l - 2.f * c + rr;
+  if (fabsf(dxx) > 1e-12f) {
+    const float o = 0.5f * (l - rr) / dxx;
+    if (fabsf(o) <= 0.5f) *kx += o;
+  }
+  const float u = shi(0, -1), dn = shi(0, 1);
+  const float dyy = u - 2.f * c + dn;
+  if (fabsf(dyy) > 1e-12f) {
+    const float o = 0.5f * (u - dn) / dyy;
+    if (fabsf(o) <= 0.5f) *ky += o;
+  }
+  return ls;
+}
+
+// Texture floor, sub-pixel parabola and cell count of each kept candidate.
+__global__ __launch_bounds__(kDetThreads) void k_det_refine(const DetDesc* __restrict__ frames) {
+  const DetDesc d = frames[blockIdx.y];
+  const int n = d.cnt[1];
+  const int W = d.W, H = d.H, r = d.r;
+  for (int j = blockIdx.x * kDetThreads + threadIdx.x; j < n; j += gridDim.x * kDetThreads) {
+    const uint32_t p = d.kept[j];
+    const int y = int(p / uint32_t(W)), x = int(p - uint32_t(y) * uint32_t(W));
+    const float c = d.resp[p];
+    float kx = float(x), ky = float(y), ls;
+    if (r == 1 && x >= 3 && y >= 3 && x + 3 < W && y + 3 < H) {
+      ls = refine_r1(d.img, W, x, y, c, d.subpixel, d.min_std, &kx, &ky);
+    } else {
+      ls = local_std7(d.img, W, H, x, y);
+      if (!(ls < d.min_std) && d.subpixel) {
+        const float l = shi_global(d.img, W, H, x - 1, y, r);
+        const float rr = shi_global(d.img, W, H, x + 1, y, r);
+        const float dxx = l - 2.f * c + rr;
+        if (fabsf(dxx) > 1e-12f) {
+          const float o = 0.5f * (l - rr) / dxx;
+          if (fabsf(o) <= 0.5f) kx += o;
         }
-        const int cell = (y / d.cell) * d.cols + (x / d.cell);
-        s = atomicAdd(&d.cell_cnt[cell], 1);
-        d.rec[i] = make_float4(kx, ky, c, ls);
+        const float u = shi_global(d.img, W, H, x, y - 1, r);
+        const float dn = shi_global(d.img, W, H, x, y + 1, r);
+        const float dyy = u - 2.f * c + dn;
+        if (fabsf(dyy) > 1e-12f) {
+          const float o = 0.5f * (u - dn) / dyy;
+          if (fabsf(o) <= 0.5f) ky += o;
+        }
       }
     }
-    d.slot[i] = s;
+    int s = -1;
+    if (!(ls < d.min_std)) {
+      s = atomicAdd(&d.cell_cnt[(y / d.cell) * d.cols + (x / d.cell)], 1);
+      d.rec[j] = make_float4(kx, ky, c, ls);
+    }
+    d.slot[j] = s;
   }
 }
 
 // Exclusive scans of the cell counts (cell lists) and of min(count, keep)
 // (output places); one workgroup per frame.
 __global__ __launch_bounds__(kFinThreads) void k_det_scan(const DetDesc* __restrict__ frames) {
-  const DetDesc& d = frames[blockIdx.x];
+  const DetDesc d = frames[blockIdx.x];
   const int nc = d.ncells, keep = d.keep;
   const int per = (nc + kFinThreads - 1) / kFinThreads;
   const int c0 = min(nc, int(threadIdx.x) * per), c1 = min(nc, c0 + per);
@@ -398,21 +779,21 @@ __global__ __launch_bounds__(kFinThreads) void k_det_scan(const DetDesc* __restr
 }
 
 __global__ __launch_bounds__(kDetThreads) void k_det_scatter(const DetDesc* __restrict__ frames) {
-  const DetDesc& d = frames[blockIdx.y];
-  const int n = d.cnt[0];
-  for (int i = blockIdx.x * kDetThreads + threadIdx.x; i < n; i += gridDim.x * kDetThreads) {
-    const int s = d.slot[i];
+  const DetDesc d = frames[blockIdx.y];
+  const int n = d.cnt[1];
+  for (int j = blockIdx.x * kDetThreads + threadIdx.x; j < n; j += gridDim.x * kDetThreads) {
+    const int s = d.slot[j];
     if (s < 0) continue;
-    const uint32_t p = d.cand[i];
+    const uint32_t p = d.kept[j];
     const int y = int(p / uint32_t(d.W)), x = int(p - uint32_t(y) * uint32_t(d.W));
-    d.items[d.cell_off[(y / d.cell) * d.cols + (x / d.cell)] + s] = i;
+    d.items[d.cell_off[(y / d.cell) * d.cols + (x / d.cell)] + s] = j;
   }
 }
 
 // One wave per cell: rank each member by (response desc, pixel index asc) and
 // write the first `keep` ranks to the output in rank order.
 __global__ __launch_bounds__(kDetThreads) void k_det_select(const DetDesc* __restrict__ frames) {
-  const DetDesc& d = frames[blockIdx.y];
+  const DetDesc d = frames[blockIdx.y];
   const int cell = blockIdx.x * (kDetThreads / kWave) + int(threadIdx.x / kWave);
   if (cell >= d.ncells) return;   // uniform over the wave
   const int lane = __lane_id();
@@ -427,7 +808,7 @@ __global__ __launch_bounds__(kDetThreads) void k_det_select(const DetDesc* __res
     if (mine) {
       ij = d.items[off + j];
       rj = d.rec[ij].z;
-      pj = d.cand[ij];
+      pj = d.kept[ij];
     }
     int rank = 0;
     for (int tb = 0; tb < m; tb += kWave) {
@@ -437,7 +818,7 @@ __global__ __launch_bounds__(kDetThreads) void k_det_select(const DetDesc* __res
       if (t < m) {
         const int it = d.items[off + t];
         rt = d.rec[it].z;
-        pt = d.cand[it];
+        pt = d.kept[it];
       }
       const int lim = min(kWave, m - tb);
       for (int u = 0; u < lim; ++u) {
@@ -598,11 +979,15 @@ DetGeom det_geom(const de_detector_cfg& c, int W, int H) {
   return g;
 }
 
+size_t det_tiles(int W, int H) {
+  return size_t((W + kNtW - 1) / kNtW) * size_t((H + kNtH - 1) / kNtH);
+}
+
 size_t det_frame_bytes(int W, int H, const DetGeom& g) {
   const size_t px = size_t(W) * size_t(H), nb = std::max<size_t>(g.region, 1);
   const size_t nc = size_t(g.ncells) + 1;
-  return a256(px) + a256(px * 4) + a256(nb * 4) + a256(nb * 16) + a256(nb * 4) +
-         a256(nb * 4) + a256(nc * 4) * 3 + a256(16);
+  return a256(px) + a256(px * 4) + a256(nb * 4) * 2 + a256(nb * 16) + a256(nb * 4) +
+         a256(nb * 4) + a256(nc * 4) * 3 + a256(16) + a256(det_tiles(W, H) * 8);
 }
 
 void det_carve(void* base, int W, int H, const DetGeom& g, DetDesc* d) {
@@ -613,6 +998,7 @@ void det_carve(void* base, int W, int H, const DetGeom& g, DetDesc* d) {
   d->state = static_cast<uint8_t*>(take(px));
   d->resp = static_cast<float*>(take(px * 4));
   d->cand = static_cast<uint32_t*>(take(nb * 4));
+  d->kept = static_cast<uint32_t*>(take(nb * 4));
   d->rec = static_cast<float4*>(take(nb * 16));
   d->slot = static_cast<int*>(take(nb * 4));
   d->items = static_cast<int*>(take(nb * 4));
@@ -620,6 +1006,7 @@ void det_carve(void* base, int W, int H, const DetGeom& g, DetDesc* d) {
   d->cell_off = static_cast<int*>(take(nc * 4));
   d->out_off = static_cast<int*>(take(nc * 4));
   d->cnt = static_cast<int*>(take(16));
+  d->tpend = static_cast<int*>(take(det_tiles(W, H) * 8));
 }
 
 void det_fill(const de_detector_cfg& c, const DetGeom& g, const uint8_t* img, int W, int H,
@@ -674,6 +1061,7 @@ int det_launch(Ctx& x, const de_detect_frame* frames, const de_detector_cfg* cfg
   p += a256(kFlags * sizeof(int));
   int maxW = 1, maxH = 1, max_cells = 1;
   size_t max_region = 1;
+  bool tiled = true;   // every frame's radius fits the LDS suppression tile
   for (int k = 0; k < n; ++k) {
     const de_detect_frame& q = frames[k];
     DetDesc& d = fr[size_t(k)];
@@ -687,7 +1075,9 @@ int det_launch(Ctx& x, const de_detect_frame* frames, const de_detector_cfg* cfg
     maxH = std::max(maxH, q.H);
     max_cells = std::max(max_cells, geo[size_t(k)].ncells);
     max_region = std::max(max_region, geo[size_t(k)].region);
+    tiled = tiled && geo[size_t(k)].rad <= kNtMaxRad;
   }
+  for (DetDesc& d : fr) d.tiled = tiled ? 1 : 0;
   Ctx::Slot* slot;
   const size_t db = fr.size() * sizeof(DetDesc);
   if (int rc = x.acquire_slot(db, &slot)) return rc;
@@ -699,15 +1089,23 @@ int det_launch(Ctx& x, const de_detect_frame* frames, const de_detector_cfg* cfg
                                                size_t(4 * cu)));
   const int cnt_blocks = std::max(1, std::min(list_blocks, (4 * cu + n - 1) / n));
   if (ev) DE_HIP(hipEventRecord(ev[0], s));
-  hipLaunchKernelGGL(k_det_init, dim3((std::max(max_cells, kFlags) + kDetThreads - 1) / kDetThreads, n),
-                     dim3(kDetThreads), 0, s, dd, flags);
+  if (!tiled)
+    hipLaunchKernelGGL(k_det_init,
+                       dim3((std::max(max_cells, kFlags) + kDetThreads - 1) / kDetThreads, n),
+                       dim3(kDetThreads), 0, s, dd, flags);
   hipLaunchKernelGGL(k_det_fast, dim3((maxW + kTileW - 1) / kTileW, (maxH + kTileH - 1) / kTileH, n),
-                     dim3(kDetThreads), 0, s, dd);
+                     dim3(kDetThreads), 0, s, dd, flags);
   DE_HIP(hipGetLastError());
   if (ev) DE_HIP(hipEventRecord(ev[1], s));
-  for (int r = 0; r < kNmsRounds; ++r)
-    hipLaunchKernelGGL(k_det_nms, dim3(cnt_blocks, n), dim3(kDetThreads), 0, s, dd, flags, r);
-  hipLaunchKernelGGL(k_det_nms_fin, dim3(n), dim3(kFinThreads), 0, s, dd, flags, kNmsRounds);
+  const int passes = tiled ? kNmsTilePasses : kNmsRounds;
+  for (int r = 0; r < passes; ++r) {
+    if (tiled)
+      hipLaunchKernelGGL(k_det_nms_tile, dim3((maxW + kNtW - 1) / kNtW, (maxH + kNtH - 1) / kNtH, n),
+                         dim3(kDetThreads), 0, s, dd, flags, r, r == passes - 1 ? 1 : 0);
+    else
+      hipLaunchKernelGGL(k_det_nms, dim3(cnt_blocks, n), dim3(kDetThreads), 0, s, dd, flags, r);
+  }
+  hipLaunchKernelGGL(k_det_nms_fin, dim3(n), dim3(kFinThreads), 0, s, dd, flags, passes);
   DE_HIP(hipGetLastError());
   if (ev) DE_HIP(hipEventRecord(ev[2], s));
   hipLaunchKernelGGL(k_det_refine, dim3(cnt_blocks, n), dim3(kDetThreads), 0, s, dd);
