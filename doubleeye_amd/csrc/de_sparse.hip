@@ -54,6 +54,7 @@ struct SpDesc {
   const float4* R;
   const uint64_t* dr;
   int nl, nr, K, mode;      // K = max(1, max_candidates)
+  int lds_dyn;              // dynamic LDS bytes of the solve launch
   float max_dy, min_d, max_d, sy;   // sy = sigma_y > 0 ? sigma_y : 1
   float w_y, w_prior, w_smooth, damp;
   int iterations;
@@ -474,93 +475,198 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
 
   // 2. message passing (match.cpp:362-407)
   float last = 1e30f;
-  for (int it = 0; it < f.iterations; ++it) {
-    float delta = 0.f;
-    // rho_ij = s_ij - max(lambda, max_{k != j} beta_ik): competition in a row.
-    // Loads are issued kChunk at a time so their latencies overlap; the
-    // arithmetic runs in ascending k as the reference's loop does.
+  // When the edge set fits, the messages live in LDS for the whole loop: rho,
+  // beta and s per edge in row-major compacted order, the column order as
+  // 16-bit compacted indices. The global form is bound by one CU's cache-line
+  // request rate (every column gather is a separate line); the arithmetic and
+  // its order are the same in both forms.
+  const size_t col_bytes = size_t(nr) * 4;
+  const size_t need = col_bytes + size_t(E) * 14 + size_t(nl + 1) * 2;
+  if (E < 65536 && need <= size_t(f.lds_dyn)) {
+    uint8_t* dyn = reinterpret_cast<uint8_t*>(col_end);
+    float* rho_s = reinterpret_cast<float*>(dyn + col_bytes);
+    float* beta_s = rho_s + E;
+    float* es_s = beta_s + E;
+    uint16_t* colq = reinterpret_cast<uint16_t*>(es_s + E);
+    uint16_t* rowst = colq + E;   // [nl + 1]
+    {
+      const int ch = (nl + kSpThreads - 1) / kSpThreads;
+      const int r0 = min(nl, tid * ch), r1 = min(nl, r0 + ch);
+      int run = 0;
+      for (int i = r0; i < r1; ++i) run += f.rc[i];
+      int tot;
+      int acc = block_excl_scan(run, shi, &tot);
+      for (int i = r0; i < r1; ++i) {
+        rowst[i] = uint16_t(acc);
+        acc += f.rc[i];
+      }
+      if (tid == 0) rowst[nl] = uint16_t(tot);
+    }
+    __syncthreads();
     for (int i = tid; i < nl; i += kSpThreads) {
-      const int n = f.rc[i], base = i * K;
-      float b1 = kNeg, b2 = kNeg;
-      int arg = -1;
-      for (int k0 = 0; k0 < n; k0 += kChunk) {
-        float v[kChunk];
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) v[u] = k0 + u < n ? f.beta[base + k0 + u] : kNeg;
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-          if (k0 + u >= n) break;
-          if (v[u] > b1) { b2 = b1; b1 = v[u]; arg = k0 + u; }
-          else if (v[u] > b2) b2 = v[u];
+      const int c0 = rowst[i], n = rowst[i + 1] - c0;
+      for (int k = 0; k < n; ++k) {
+        rho_s[c0 + k] = 0.f;
+        beta_s[c0 + k] = 0.f;
+        es_s[c0 + k] = f.es[i * K + k];
+      }
+    }
+    for (int q = tid; q < E; q += kSpThreads) {
+      const int e = f.col_edges[q];
+      const int i = e / K;
+      colq[q] = uint16_t(rowst[i] + (e - i * K));
+    }
+    __syncthreads();
+    for (int it = 0; it < f.iterations; ++it) {
+      float delta = 0.f;
+      for (int i = tid; i < nl; i += kSpThreads) {
+        const int c0 = rowst[i], n = rowst[i + 1] - c0;
+        float b1 = kNeg, b2 = kNeg;
+        int arg = -1;
+        for (int k = 0; k < n; ++k) {
+          const float v = beta_s[c0 + k];
+          if (v > b1) { b2 = b1; b1 = v; arg = k; }
+          else if (v > b2) b2 = v;
+        }
+        for (int k = 0; k < n; ++k) {
+          const float other = k == arg ? b2 : b1;
+          const float target = es_s[c0 + k] - fmax_ref(f.lambda, other);
+          const float ov = rho_s[c0 + k];
+          const float nv = (1.f - f.damp) * target + f.damp * ov;
+          delta = fmax_ref(delta, fabsf(nv - ov));
+          rho_s[c0 + k] = nv;
         }
       }
-      for (int k0 = 0; k0 < n; k0 += kChunk) {
-        float sv[kChunk], ov[kChunk];
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-          sv[u] = k0 + u < n ? f.es[base + k0 + u] : 0.f;
-          ov[u] = k0 + u < n ? f.rho[base + k0 + u] : 0.f;
+      __syncthreads();
+      for (int j = tid; j < nr; j += kSpThreads) {
+        const int q0 = col_begin(j), q1 = col_end[j];
+        float b1 = kNeg, b2 = kNeg;
+        int arg = -1;
+        for (int q = q0; q < q1; ++q) {
+          const int c = colq[q];
+          const float v = rho_s[c];
+          if (v > b1) { b2 = b1; b1 = v; arg = c; }
+          else if (v > b2) b2 = v;
         }
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-          if (k0 + u >= n) break;
-          const float other = k0 + u == arg ? b2 : b1;
-          const float target = sv[u] - fmax_ref(f.lambda, other);
-          const float nv = (1.f - f.damp) * target + f.damp * ov[u];
-          delta = fmax_ref(delta, fabsf(nv - ov[u]));
-          f.rho[base + k0 + u] = nv;
+        for (int q = q0; q < q1; ++q) {
+          const int c = colq[q];
+          const float other = c == arg ? b2 : b1;
+          const float target = es_s[c] - fmax_ref(f.gamma, other);
+          const float ov = beta_s[c];
+          const float nv = (1.f - f.damp) * target + f.damp * ov;
+          delta = fmax_ref(delta, fabsf(nv - ov));
+          beta_s[c] = nv;
         }
+      }
+      delta = block_max(delta, shf);   // its barriers also order the phases
+      if (delta > last) ++st.oscillating;
+      last = delta;
+      st.iterations_run = it + 1;
+      st.final_max_delta = delta;
+      if (delta < f.eps) {
+        st.converged = 1;
+        break;
       }
     }
     __syncthreads();
-    // beta_ij = s_ij - max(gamma, max_{k != i} rho_kj): competition in a column
-    for (int j = tid; j < nr; j += kSpThreads) {
-      const int q0 = col_begin(j), q1 = col_end[j];
-      float b1 = kNeg, b2 = kNeg;
-      int arg = -1;
-      for (int c0 = q0; c0 < q1; c0 += kChunk) {
-        int ev[kChunk];
-        float v[kChunk];
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) ev[u] = c0 + u < q1 ? f.col_edges[c0 + u] : 0;
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) v[u] = c0 + u < q1 ? f.rho[ev[u]] : kNeg;
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-          if (c0 + u >= q1) break;
-          if (v[u] > b1) { b2 = b1; b1 = v[u]; arg = ev[u]; }
-          else if (v[u] > b2) b2 = v[u];
-        }
-      }
-      for (int c0 = q0; c0 < q1; c0 += kChunk) {
-        int ev[kChunk];
-        float sv[kChunk], ov[kChunk];
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) ev[u] = c0 + u < q1 ? f.col_edges[c0 + u] : 0;
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-          sv[u] = c0 + u < q1 ? f.es[ev[u]] : 0.f;
-          ov[u] = c0 + u < q1 ? f.beta[ev[u]] : 0.f;
-        }
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-          if (c0 + u >= q1) break;
-          const float other = ev[u] == arg ? b2 : b1;
-          const float target = sv[u] - fmax_ref(f.gamma, other);
-          const float nv = (1.f - f.damp) * target + f.damp * ov[u];
-          delta = fmax_ref(delta, fabsf(nv - ov[u]));
-          f.beta[ev[u]] = nv;
-        }
+    for (int i = tid; i < nl; i += kSpThreads) {
+      const int c0 = rowst[i], n = rowst[i + 1] - c0;
+      for (int k = 0; k < n; ++k) {
+        f.rho[i * K + k] = rho_s[c0 + k];
+        f.beta[i * K + k] = beta_s[c0 + k];
       }
     }
-    delta = block_max(delta, shf);
-    if (delta > last) ++st.oscillating;
-    last = delta;
-    st.iterations_run = it + 1;
-    st.final_max_delta = delta;
-    if (delta < f.eps) {
-      st.converged = 1;
-      break;
+    __syncthreads();
+    // col_end's region was not touched; the decision below reads it
+  } else {
+    for (int it = 0; it < f.iterations; ++it) {
+      float delta = 0.f;
+      // rho_ij = s_ij - max(lambda, max_{k != j} beta_ik): competition in a row.
+      // Loads are issued kChunk at a time so their latencies overlap; the
+      // arithmetic runs in ascending k as the reference's loop does.
+      for (int i = tid; i < nl; i += kSpThreads) {
+        const int n = f.rc[i], base = i * K;
+        float b1 = kNeg, b2 = kNeg;
+        int arg = -1;
+        for (int k0 = 0; k0 < n; k0 += kChunk) {
+          float v[kChunk];
+  #pragma unroll
+          for (int u = 0; u < kChunk; ++u) v[u] = k0 + u < n ? f.beta[base + k0 + u] : kNeg;
+  #pragma unroll
+          for (int u = 0; u < kChunk; ++u) {
+            if (k0 + u >= n) break;
+            if (v[u] > b1) { b2 = b1; b1 = v[u]; arg = k0 + u; }
+            else if (v[u] > b2) b2 = v[u];
+          }
+        }
+        for (int k0 = 0; k0 < n; k0 += kChunk) {
+          float sv[kChunk], ov[kChunk];
+  #pragma unroll
+          for (int u = 0; u < kChunk; ++u) {
+            sv[u] = k0 + u < n ? f.es[base + k0 + u] : 0.f;
+            ov[u] = k0 + u < n ? f.rho[base + k0 + u] : 0.f;
+          }
+  #pragma unroll
+          for (int u = 0; u < kChunk; ++u) {
+            if (k0 + u >= n) break;
+            const float other = k0 + u == arg ? b2 : b1;
+            const float target = sv[u] - fmax_ref(f.lambda, other);
+            const float nv = (1.f - f.damp) * target + f.damp * ov[u];
+            delta = fmax_ref(delta, fabsf(nv - ov[u]));
+            f.rho[base + k0 + u] = nv;
+          }
+        }
+      }
+      __syncthreads();
+      // beta_ij = s_ij - max(gamma, max_{k != i} rho_kj): competition in a column
+      for (int j = tid; j < nr; j += kSpThreads) {
+        const int q0 = col_begin(j), q1 = col_end[j];
+        float b1 = kNeg, b2 = kNeg;
+        int arg = -1;
+        for (int c0 = q0; c0 < q1; c0 += kChunk) {
+          int ev[kChunk];
+          float v[kChunk];
+  #pragma unroll
+          for (int u = 0; u < kChunk; ++u) ev[u] = c0 + u < q1 ? f.col_edges[c0 + u] : 0;
+  #pragma unroll
+          for (int u = 0; u < kChunk; ++u) v[u] = c0 + u < q1 ? f.rho[ev[u]] : kNeg;
+  #pragma unroll
+          for (int u = 0; u < kChunk; ++u) {
+            if (c0 + u >= q1) break;
+            if (v[u] > b1) { b2 = b1; b1 = v[u]; arg = ev[u]; }
+            else if (v[u] > b2) b2 = v[u];
+          }
+        }
+        for (int c0 = q0; c0 < q1; c0 += kChunk) {
+          int ev[kChunk];
+          float sv[kChunk], ov[kChunk];
+  #pragma unroll
+          for (int u = 0; u < kChunk; ++u) ev[u] = c0 + u < q1 ? f.col_edges[c0 + u] : 0;
+  #pragma unroll
+          for (int u = 0; u < kChunk; ++u) {
+            sv[u] = c0 + u < q1 ? f.es[ev[u]] : 0.f;
+            ov[u] = c0 + u < q1 ? f.beta[ev[u]] : 0.f;
+          }
+  #pragma unroll
+          for (int u = 0; u < kChunk; ++u) {
+            if (c0 + u >= q1) break;
+            const float other = ev[u] == arg ? b2 : b1;
+            const float target = sv[u] - fmax_ref(f.gamma, other);
+            const float nv = (1.f - f.damp) * target + f.damp * ov[u];
+            delta = fmax_ref(delta, fabsf(nv - ov[u]));
+            f.beta[ev[u]] = nv;
+          }
+        }
+      }
+      delta = block_max(delta, shf);
+      if (delta > last) ++st.oscillating;
+      last = delta;
+      st.iterations_run = it + 1;
+      st.final_max_delta = delta;
+      if (delta < f.eps) {
+        st.converged = 1;
+        break;
+      }
     }
   }
   __syncthreads();
@@ -760,13 +866,31 @@ size_t max_sort_lds(const std::vector<SpDesc>& fr) {
   return m;
 }
 
-size_t max_solve_lds(const std::vector<SpDesc>& fr) {
-  size_t m = 4;
-  for (const SpDesc& d : fr) m = std::max(m, size_t(d.nr) * 4);
-  return m;
-}
-
 }  // namespace
+
+// Dynamic LDS of the solve launch: everything the device allows beyond the
+// kernel's static arrays (the LDS-resident message loop uses it when the edge
+// set fits; the column ends need 4 bytes per right keypoint in any case).
+static int solve_dyn_lds(int* bytes) {
+  static int cached = -1;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  if (cached < 0) {
+    hipFuncAttributes attr;
+    DE_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(k_sp_solve)));
+    int dev = 0, lds = 0;
+    DE_HIP(hipGetDevice(&dev));
+    DE_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const int dyn = lds - int(attr.sharedSizeBytes);
+    if (dyn < 4 * DE_SPARSE_MAX_RIGHT)
+      return fail(DE_ERR_UNSUPPORTED, "sparse solve: %d bytes of LDS left", dyn);
+    DE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sp_solve),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    cached = dyn;
+  }
+  *bytes = cached;
+  return DE_OK;
+}
 
 // Launch the three stages over frames whose descriptors are at d_frames (device)
 // with left-keypoint prefix d_start; `fr` is the host copy.
@@ -785,7 +909,9 @@ static int sp_launch(const std::vector<SpDesc>& fr, const SpDesc* d_frames, cons
     DE_HIP(hipGetLastError());
   }
   if (solve) {
-    hipLaunchKernelGGL(k_sp_solve, dim3(n), dim3(kSpThreads), max_solve_lds(fr), s, d_frames);
+    int dyn = 0;
+    if (int rc = solve_dyn_lds(&dyn)) return rc;
+    hipLaunchKernelGGL(k_sp_solve, dim3(n), dim3(kSpThreads), size_t(dyn), s, d_frames);
     DE_HIP(hipGetLastError());
   }
   return DE_OK;
@@ -797,9 +923,13 @@ static int sp_run(Ctx& x, const std::vector<SpDesc>& fr, bool solve, hipStream_t
   std::vector<int> start(size_t(n) + 1, 0);
   for (int k = 0; k < n; ++k) start[size_t(k) + 1] = start[size_t(k)] + fr[size_t(k)].nl;
   const size_t db = fr.size() * sizeof(SpDesc), sb = start.size() * sizeof(int);
+  int dyn = 0;
+  if (solve)
+    if (int rc = solve_dyn_lds(&dyn)) return rc;
   Ctx::Slot* slot;
   if (int rc = x.acquire_slot(db + sb, &slot)) return rc;
   std::memcpy(slot->host, fr.data(), db);
+  for (int k = 0; k < n; ++k) static_cast<SpDesc*>(slot->host)[k].lds_dyn = dyn;
   std::memcpy(static_cast<uint8_t*>(slot->host) + db, start.data(), sb);
   DE_HIP(hipMemcpyAsync(slot->dev, slot->host, db + sb, hipMemcpyHostToDevice, s));
   const SpDesc* d_frames = static_cast<const SpDesc*>(slot->dev);
