@@ -36,6 +36,7 @@ SOURCES = {
     "de_fp32.hip": ["-ffp-contract=off"],
     "de_stream.hip": [],
     "de_detect.hip": ["-ffp-contract=off"],
+    "de_prior.hip": [],
 }
 
 

@@ -382,7 +382,7 @@ Ctx::~Ctx() {
       for (int i = 0; i <= kNumStages; ++i) (void)hipEventDestroy(ev.e[i]);
   for (DevBuf* b : {&scratch, &rf_buf, &sc_buf, &io_in, &io_out, &sp_scratch, &sp_io,
                     &kb_scratch, &kb_io, &ed_scratch, &ed_io, &f32_scratch, &f32_io,
-                    &det_scratch, &det_io, &pipe_io})
+                    &det_scratch, &det_io, &pipe_io, &pr_buf, &pr_io})
     b->release();
   if (stream) (void)hipStreamDestroy(stream);
 }

@@ -64,6 +64,7 @@ struct Ctx {
   DevBuf f32_scratch, f32_io; // FP32 full-volume path: scratch, staged inputs/outputs
   DevBuf det_scratch, det_io; // keypoint detector: scratch, staged images and keypoints
   DevBuf pipe_io;             // detect -> match pipeline: descriptors and matches
+  DevBuf pr_buf, pr_io;       // prior-masked sweep: mask planes, prior / coarse level
   Slot slots[kSlots];
   int next_slot = 0;
   bool store_vol = false;
@@ -123,6 +124,9 @@ hipError_t launch_lrc_exact(const BatchView& b, int items, hipStream_t s);
 hipError_t launch_lrc_plane(const BatchView& b, int items, hipStream_t s);
 hipError_t launch_census_plane(const uint8_t* img, int W, int H, int hw, int hh,
                                uint64_t* out, hipStream_t s);
+
+hipError_t launch_masked(const FrameDesc& hf, const FrameDesc* df, const float* prior, int band,
+                         const MaskBufs& m, size_t lds_row, bool lrc, hipStream_t s);
 
 // keypoint-graph BP over frames whose scratch blocks (knn_scratch_bytes each)
 // the caller provides; launches on s without synchronising

@@ -70,6 +70,15 @@ struct FrameDesc {
   float lambda, gamma, damping, min_margin, min_ratio;
 };
 
+// Mask planes of the prior-masked sweep (--prior / --c2f) of one frame.
+struct MaskBufs {
+  int* dc;      // [H][W] rounded prior
+  int* mx0;     // [H][D] first x of each row's interval (W = none)
+  int* mx1;     // [H][D] last x (-1 = none)
+  int4* box;    // [D] bounding box of the intervals: ylo, yhi, xlo, xhi (inclusive)
+  int4* rect;   // [D] filtered rectangle y0, y1, x0, x1 (half-open); y1 <= y0: skipped
+};
+
 // Packed top-2 key: ((v' + 1) << 8) | (255 - k). Integer > orders by value
 // descending then k ascending, so the reduction is order independent and
 // reproduces the sequential ascending-k strictly-greater insertion

@@ -614,4 +614,248 @@ hipError_t launch_lrc_plane(const BatchView& b, int items, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Prior-masked sweep (--prior / --c2f, core/tools/de_dense.cpp:797-853,
+// 1156-1296). Plane k is evaluated only on its rectangle: the bounding box of
+// the pixels whose band |d - round(prior)| <= band admits it, padded by 8 and
+// clipped to the score region. All four filter passes are restricted to that
+// rectangle and start from its edge values (the reference's rect filter), and
+// a score enters the candidates only inside its row's interval. Lane = plane,
+// as in the full sweep; lanes whose plane does not cover the row (column)
+// idle through it.
+
+// Rounded prior per pixel, clamped so that dc +- band cannot overflow.
+__global__ __launch_bounds__(256) void k_mask_dc(const float* __restrict__ prior, int n,
+                                                 int* __restrict__ dc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float r = roundf(prior[i]);   // std::lround: halves away from zero
+  dc[i] = r >= 1e9f ? 1000000000 : (r <= -1e9f ? -1000000000 : int(r));
+}
+
+// One thread per (row, plane): the row's interval [mx0, mx1] of pixels whose
+// band admits the plane (mx0 = W, mx1 = -1 when none).
+__global__ __launch_bounds__(256) void k_mask_iv(const FrameDesc* __restrict__ fp,
+                                                 const int* __restrict__ dc, int band,
+                                                 int* __restrict__ mx0, int* __restrict__ mx1) {
+  const FrameDesc F = *fp;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= F.H * F.D) return;
+  const int y = t / F.D, k = t - y * F.D;
+  const long long d = F.dmin + k;
+  const int* row = dc + size_t(y) * F.W;
+  int a = F.W, b = -1;
+  for (int x = 0; x < F.W; ++x) {
+    const long long c = row[x];
+    if (d >= c - band && d <= c + band) {
+      a = min(a, x);
+      b = x;
+    }
+  }
+  mx0[t] = a;
+  mx1[t] = b;
+}
+
+// One thread per plane: the bounding box of its intervals and the filtered
+// rectangle (y0, y1, x0, x1 half-open; y1 = y0 = 0 when the plane is skipped).
+__global__ __launch_bounds__(256) void k_mask_rect(const FrameDesc* __restrict__ fp,
+                                                   const int* __restrict__ mx0,
+                                                   const int* __restrict__ mx1,
+                                                   int4* __restrict__ box, int4* __restrict__ rect) {
+  const FrameDesc F = *fp;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= F.D) return;
+  int ylo = F.H, yhi = -1, xlo = F.W, xhi = -1;
+  for (int y = 0; y < F.H; ++y) {
+    const int a = mx0[size_t(y) * F.D + k], b = mx1[size_t(y) * F.D + k];
+    if (b < a) continue;
+    ylo = min(ylo, y);
+    yhi = y;
+    xlo = min(xlo, a);
+    xhi = max(xhi, b);
+  }
+  constexpr int kPad = 8;   // MPAD: real scores this far outside the intervals
+  const int d = F.dmin + k;
+  const int y0 = max(F.hh, ylo - kPad), y1 = min(F.H - F.hh, yhi + 1 + kPad);
+  const int x0 = max(F.hw + d, xlo - kPad), x1 = min(F.W - F.hw, xhi + 1 + kPad);
+  const bool skip = yhi < ylo || y1 - y0 < 1 || x1 - x0 < 2;
+  box[k] = make_int4(ylo, yhi, xlo, xhi);
+  rect[k] = skip ? make_int4(0, 0, 0, 0) : make_int4(y0, y1, x0, x1);
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+  for (int o = 1; o < kWave; o <<= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 1; o < kWave; o <<= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Score + H-fwd + H-bwd of one row and 64 planes, each plane on its own
+// rectangle's x range.
+__global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
+                                             const int4* __restrict__ rect) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const FrameDesc F = *fp;
+  const int W = F.W, Dp = F.Dpad, G = F.G;
+  const int y = blockIdx.x / G, g = blockIdx.x - y * G;
+  const int lane = int(threadIdx.x);
+  const int k = g * kWave + lane;
+  const int d = F.dmin + k;
+  const int4 r = k < F.D ? rect[k] : make_int4(0, 0, 0, 0);
+  const bool act = k < F.D && y >= r.x && y < r.y;
+  const int xs = act ? r.z : W, xe = act ? r.w : 0;
+  const int xmin = wave_min(xs), xmax = wave_max(xe);
+  if (xmin >= xmax) return;   // uniform over the wave
+  const int nb = (2 * F.hw + 1) * (2 * F.hh + 1) - 1;
+  const int ntbl = (nb + 1) << F.sc_shift;
+  uint4* lr = reinterpret_cast<uint4*>(smem + ((ntbl * 2 + 15) & ~15));
+  uint64_t* crs = reinterpret_cast<uint64_t*>(lr + W);
+  uint16_t* r2s = reinterpret_cast<uint16_t*>(crs + W);
+  {
+    uint16_t* tbl = reinterpret_cast<uint16_t*>(smem);
+    for (int i = lane; i < ntbl; i += kWave) tbl[i] = F.sc[i];
+    const uint4* lrg = reinterpret_cast<const uint4*>(F.lrec) + size_t(y) * W;
+    const uint64_t* crg = F.cr + size_t(y) * W;
+    const uint8_t* Rg = F.R + size_t(y) * W;
+    for (int x = lane; x < W; x += kWave) {
+      lr[x] = lrg[x];
+      crs[x] = crg[x];
+      r2s[x] = uint16_t(2 * Rg[x]);
+    }
+  }
+  __syncthreads();
+  const int sh = F.sc_shift + 1;
+  const uint32_t T2 = 2u * uint32_t(F.ad_T);
+  uint16_t* const x0p = F.vol + size_t(y) * W * Dp + g * kWave;
+  uint32_t t2 = 0u;
+  for (int x = xmin; x < xmax; ++x) {
+    const uint4 rec = lr[x];
+    if (x >= xs && x < xe) {
+      const uint32_t c = score_at(smem, crs, r2s, rec, x - d, sh, T2);
+      t2 = x == xs ? (c << 16) : rf_step_rec(rec, t2, c);
+      x0p[size_t(x) * Dp + lane] = hi16(t2);
+    }
+  }
+  for (int x = xmax - 2; x >= xmin; --x) {
+    const uint4 rec = lr[x + 1];
+    if (x >= xs && x < xe - 1) {
+      const uint32_t c = x0p[size_t(x) * Dp + lane];
+      t2 = rf_step_rec(rec, t2, c);
+      x0p[size_t(x) * Dp + lane] = hi16(t2);
+    }
+  }
+}
+
+// V-fwd + V-bwd of one column and 64 planes, each plane on its own rectangle's
+// y range; the final values are stored back into the volume.
+__global__ __launch_bounds__(64) void k_mcol(const FrameDesc* __restrict__ fp,
+                                             const int4* __restrict__ rect) {
+  const FrameDesc F = *fp;
+  const int W = F.W, H = F.H, Dp = F.Dpad, G = F.G;
+  const int x = blockIdx.x / G, g = blockIdx.x - x * G;
+  const int lane = int(threadIdx.x);
+  const int k = g * kWave + lane;
+  const int4 r = k < F.D ? rect[k] : make_int4(0, 0, 0, 0);
+  const bool act = k < F.D && x >= r.z && x < r.w && r.y > r.x;
+  const int ys = act ? r.x : H, ye = act ? r.y : 0;
+  const int ymin = wave_min(ys), ymax = wave_max(ye);
+  if (ymin >= ymax) return;   // uniform over the wave
+  const size_t rs = size_t(W) * Dp;
+  uint16_t* const col = F.vol + size_t(x) * Dp + g * kWave + lane;
+  const uint32_t* ay = F.ayt + size_t(x) * H;
+  uint32_t t = 0u;
+  for (int y = ymin; y < ymax; ++y) {
+    const uint32_t a2 = ay[y];
+    if (y >= ys && y < ye) {
+      const uint32_t c = col[size_t(y) * rs];
+      if (y == ys) {
+        t = c << 16;
+      } else {
+        t = rf_chain_v(a2, t, rf_acc(a2, c));
+        col[size_t(y) * rs] = hi16(t);
+      }
+    }
+  }
+  for (int y = ymax - 2; y >= ymin; --y) {
+    const uint32_t a2 = ay[y + 1];
+    if (y >= ys && y < ye - 1) {
+      const uint32_t c = col[size_t(y) * rs];
+      t = rf_chain_v(a2, t, rf_acc(a2, c));
+      col[size_t(y) * rs] = hi16(t);
+    }
+  }
+}
+
+// Plane k's score at (x, y) is a candidate: inside the rectangle, the plane's
+// row range and the row's interval.
+__device__ __forceinline__ bool mask_inserts(const int4& r, const int4& b, const int* mx0,
+                                             const int* mx1, int D, int y, int x, int k) {
+  if (r.y <= r.x) return false;
+  if (y < max(r.x, b.x) || y >= min(r.y, b.y + 1)) return false;
+  const int xa = max(r.z, mx0[size_t(y) * D + k]);
+  const int xb = min(r.w, mx1[size_t(y) * D + k] + 1);
+  return x >= xa && x < xb;
+}
+
+// Top-2 per pixel over the inserted planes (ascending k, strictly greater
+// displaces: the packed keys), no sub-pixel neighbours.
+__global__ __launch_bounds__(256) void k_minsert(const FrameDesc* __restrict__ fp,
+                                                 const int4* __restrict__ rect,
+                                                 const int4* __restrict__ box,
+                                                 const int* __restrict__ mx0,
+                                                 const int* __restrict__ mx1) {
+  const FrameDesc F = *fp;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= F.W * F.H) return;
+  const int y = i / F.W, x = i - y * F.W;
+  uint32_t p0 = 0u, p1 = 0u;
+  for (int k = 0; k < F.D; ++k) {
+    if (!mask_inserts(rect[k], box[k], mx0, mx1, F.D, y, x, k)) continue;
+    const uint32_t key = pack_key(F.vol[size_t(i) * F.Dpad + k], k);
+    p1 = med3(p0, p1, key);
+    p0 = max(p0, key);
+  }
+  reinterpret_cast<uint4*>(F.cand)[i] = make_uint4(p0, p1, 0xFFFFFFFFu, 0u);
+}
+
+// Exact reverse match over the inserted scores only.
+__global__ __launch_bounds__(256) void k_mlrc(const FrameDesc* __restrict__ fp,
+                                              const int4* __restrict__ rect,
+                                              const int4* __restrict__ box,
+                                              const int* __restrict__ mx0,
+                                              const int* __restrict__ mx1) {
+  const FrameDesc F = *fp;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= F.W * F.H) return;
+  const int y = j / F.W, xr = j - y * F.W;
+  uint32_t key = 0u;
+  for (int k = 0; k < F.D; ++k) {
+    const int x = xr + F.dmin + k;
+    if (x >= F.W) break;
+    if (!mask_inserts(rect[k], box[k], mx0, mx1, F.D, y, x, k)) continue;
+    key = max(key, pack_key(F.vol[(size_t(y) * F.W + x) * F.Dpad + k], k));
+  }
+  F.rdisp[j] = key ? int16_t(key_k(key)) : int16_t(-1);
+}
+
+hipError_t launch_masked(const FrameDesc& hf, const FrameDesc* df, const float* prior, int band,
+                         const MaskBufs& m, size_t lds_row, bool lrc, hipStream_t s) {
+  const int WH = hf.W * hf.H, HD = hf.H * hf.D;
+  hipLaunchKernelGGL(k_mask_dc, dim3((WH + 255) / 256), dim3(256), 0, s, prior, WH, m.dc);
+  hipLaunchKernelGGL(k_mask_iv, dim3((HD + 255) / 256), dim3(256), 0, s, df, m.dc, band, m.mx0,
+                     m.mx1);
+  hipLaunchKernelGGL(k_mask_rect, dim3((hf.D + 255) / 256), dim3(256), 0, s, df, m.mx0, m.mx1,
+                     m.box, m.rect);
+  hipLaunchKernelGGL(k_mrow, dim3(hf.H * hf.G), dim3(64), lds_row, s, df, m.rect);
+  hipLaunchKernelGGL(k_mcol, dim3(hf.W * hf.G), dim3(64), 0, s, df, m.rect);
+  hipLaunchKernelGGL(k_minsert, dim3((WH + 255) / 256), dim3(256), 0, s, df, m.rect, m.box,
+                     m.mx0, m.mx1);
+  if (lrc)
+    hipLaunchKernelGGL(k_mlrc, dim3((WH + 255) / 256), dim3(256), 0, s, df, m.rect, m.box,
+                       m.mx0, m.mx1);
+  return hipGetLastError();
+}
+
 }  // namespace de

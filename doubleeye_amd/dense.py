@@ -208,6 +208,41 @@ class Context:
                                  _p(cand) if cand is not None else None))
         return disp, margin, cand
 
+    def run_dense_prior(self, L, R, prior, cfg: Cfg | None = None, band: int = 2,
+                        want_cand: bool = False):
+        """`de_dense --prior F --band N`: (disp, margin, cand6 or None). `prior` is
+        H x W, or H/2 x W/2 (upsampled nearest, disparity doubled)."""
+        cfg = cfg or Cfg()
+        L = np.ascontiguousarray(L, np.uint8)
+        R = np.ascontiguousarray(R, np.uint8)
+        H, W = L.shape
+        pr = np.ascontiguousarray(prior, np.float32)
+        disp = np.empty((H, W), np.float32)
+        margin = np.empty((H, W), np.float32)
+        cand = np.empty((6, H, W), np.float32) if want_cand else None
+        check(lib().de_dense_run_prior(self.h, _p(L), _p(R), W, H, C.byref(cfg.to_c()), _p(pr),
+                                       pr.size, int(band), _p(disp), _p(margin),
+                                       _p(cand) if cand is not None else None))
+        return disp, margin, cand
+
+    def run_dense_c2f(self, L, R, cfg: Cfg | None = None, band: int = 2,
+                      want_cand: bool = False):
+        """`de_dense --c2f --band N`: (disp, margin, cand6 or None, the filled
+        coarse map, whether the masked sweep ran)."""
+        cfg = cfg or Cfg()
+        L = np.ascontiguousarray(L, np.uint8)
+        R = np.ascontiguousarray(R, np.uint8)
+        H, W = L.shape
+        disp = np.empty((H, W), np.float32)
+        margin = np.empty((H, W), np.float32)
+        cand = np.empty((6, H, W), np.float32) if want_cand else None
+        coarse = np.empty((H // 2, W // 2), np.float32)
+        used = C.c_int(0)
+        check(lib().de_dense_run_c2f(self.h, _p(L), _p(R), W, H, C.byref(cfg.to_c()), int(band),
+                                     _p(disp), _p(margin), _p(cand) if cand is not None else None,
+                                     _p(coarse), C.byref(used)))
+        return disp, margin, cand, coarse, bool(used.value)
+
     def volume(self, L, R, cfg: Cfg, k0: int, k1: int) -> np.ndarray:
         L = np.ascontiguousarray(L, np.uint8)
         R = np.ascontiguousarray(R, np.uint8)

@@ -276,6 +276,29 @@ typedef struct de_sparse_frame {
 int de_match_masda_batch_dev(de_ctx* ctx, const de_sparse_frame* frames,
                              const de_match_cfg* cfgs, int n, void* stream);
 
+/* ==== The prior as a mask (--prior, --c2f) =================================
+ * core/tools/de_dense.cpp:797-853, 1156-1296, 1771-1966. Plane d is evaluated
+ * on the rectangle bounding the pixels whose band |d - round(prior)| <= band
+ * admits it (padded by 8, clipped to the score region); the recursive filter
+ * runs on that rectangle only, from its own edge values, and a score becomes a
+ * candidate only inside its row's interval. Sub-pixel is off (the reference
+ * turns it off under a mask). Outputs as de_dense_run. */
+
+/* --prior: `prior` holds W*H values, or W/2 * H/2 (upsampled nearest, the
+ * disparity doubled, as the reference does for a half-resolution prior); every
+ * value finite (DE_ERR_ARG otherwise). band: the reference's --band (2). */
+int de_dense_run_prior(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+                       const de_cfg* cfg, const float* prior, size_t n_prior, int band,
+                       float* disp, float* margin, float* cand6);
+
+/* --c2f: the half-resolution pass (dmin/2 .. (dmax+1)/2, one iteration,
+ * min_margin 0), hole fill, upsample, then the masked sweep; the full sweep
+ * when holes remain. coarse (nullable) gets the filled W/2 x H/2 coarse map,
+ * *used_mask (nullable) 1 when the masked sweep ran. */
+int de_dense_run_c2f(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
+                     const de_cfg* cfg, int band, float* disp, float* margin, float* cand6,
+                     float* coarse, int* used_mask);
+
 /* ==== Keypoint detector and the detect -> match pipeline ===================
  * detect_keypoints_fast (core/src/preproc.cpp:436-545): FAST-9 candidates,
  * Shi-Tomasi response at candidates, greedy suppression within nms_radius in

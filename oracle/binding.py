@@ -137,6 +137,52 @@ def dense_run(L, R, c: OrcCfg, want_cand: bool = False, want_ties: bool = False)
     return disp, margin, cand, ties
 
 
+def dense_run_prior(L, R, c: OrcCfg, prior, band: int = 2, want_cand: bool = False,
+                    want_ties: bool = False):
+    """--prior: (disp, margin, cand6 or None, tie_mask or None). `prior` is
+    W*H, or W/2 x H/2 (nearest-upsampled, doubled)."""
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    pr = np.ascontiguousarray(prior, np.float32)
+    if pr.size == (W // 2) * (H // 2) and pr.size != W * H:
+        full = np.empty((H, W), np.float32)
+        lib().orc_prior_upsample(_p(pr), W, H, _p(full))
+        pr = full
+    pr = pr.reshape(H, W)
+    disp = np.empty((H, W), np.float32)
+    margin = np.empty((H, W), np.float32)
+    cand = np.empty((6, H, W), np.float32) if want_cand else None
+    ties = np.zeros((H, W), np.uint8) if want_ties else None
+    rc = lib().orc_dense_run_prior(_p(L), _p(R), W, H, C.byref(c), _p(pr), int(band), _p(disp),
+                                   _p(margin), _p(cand) if cand is not None else None,
+                                   _p(ties) if ties is not None else None)
+    if rc:
+        raise ValueError(f"orc_dense_run_prior rc={rc}")
+    return disp, margin, cand, ties
+
+
+def dense_run_c2f(L, R, c: OrcCfg, band: int = 2, want_cand: bool = False,
+                  want_ties: bool = False):
+    """--c2f: (disp, margin, cand6 or None, tie_mask or None, coarse, used_mask)."""
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    disp = np.empty((H, W), np.float32)
+    margin = np.empty((H, W), np.float32)
+    cand = np.empty((6, H, W), np.float32) if want_cand else None
+    ties = np.zeros((H, W), np.uint8) if want_ties else None
+    coarse = np.empty((H // 2, W // 2), np.float32)
+    used = C.c_int(0)
+    rc = lib().orc_dense_run_c2f(_p(L), _p(R), W, H, C.byref(c), int(band), _p(disp),
+                                 _p(margin), _p(cand) if cand is not None else None,
+                                 _p(ties) if ties is not None else None, _p(coarse),
+                                 C.byref(used))
+    if rc:
+        raise ValueError(f"orc_dense_run_c2f rc={rc}")
+    return disp, margin, cand, ties, coarse, bool(used.value)
+
+
 def dense_run_full(L, R, c: OrcCfg, want_ties: bool = False):
     """--topk 0: (disp, margin, tie_mask or None)."""
     L = np.ascontiguousarray(L, np.uint8)

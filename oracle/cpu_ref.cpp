@@ -327,6 +327,69 @@ void solve_row_k2(int W, const OrcCfg& cfg, const float* cs, const int* cd,
   }
 }
 
+// Candidates -> solve_row_sparse per row, confidence planes, LRC plane
+// (de_dense.cpp:1498-1597): the part after the insert, shared by the full and
+// the masked sweeps.
+void finish_dense(int W, int H, const OrcCfg& cfg, const std::vector<int16_t>& s0,
+                  const std::vector<int16_t>& s1, const std::vector<int16_t>& d0,
+                  const std::vector<int16_t>& d1, const std::vector<int16_t>& m0,
+                  const std::vector<int16_t>& p0, const std::vector<int16_t>& rd, float* disp,
+                  float* margin, float* cand6, uint8_t* tie_mask) {
+  const size_t WH = size_t(W) * H;
+  const float q = 1.f / float(kOne);
+  std::vector<float> cs(size_t(W) * 2), cnb(size_t(W) * 2), lrc2(W);
+  std::vector<int> cd(size_t(W) * 2), cn(W), tie(W);
+  for (int y = 0; y < H; ++y) {
+    for (int x = 0; x < W; ++x) {
+      const size_t i = size_t(y) * W + x;
+      int n = 0;
+      cs[2 * x] = cs[2 * x + 1] = kNone;
+      cd[2 * x] = cd[2 * x + 1] = -1;
+      cnb[2 * x] = cnb[2 * x + 1] = kNone;
+      if (s0[i] > -32768) {
+        cs[2 * x] = float(s0[i]) * q; cd[2 * x] = d0[i]; n = 1;
+        if (cfg.subpixel) {
+          cnb[2 * x] = (m0[i] == -32768) ? kNone : float(m0[i]) * q;
+          cnb[2 * x + 1] = (p0[i] == -32768) ? kNone : float(p0[i]) * q;
+        }
+        if (s1[i] > -32768) { cs[2 * x + 1] = float(s1[i]) * q; cd[2 * x + 1] = d1[i]; n = 2; }
+      }
+      cn[x] = n;
+    }
+    solve_row_k2(W, cfg, cs.data(), cd.data(), cn.data(),
+                 cfg.subpixel ? cnb.data() : nullptr, disp + size_t(y) * W,
+                 margin + size_t(y) * W, cand6 ? lrc2.data() : nullptr,
+                 tie_mask ? tie.data() : nullptr);
+    if (tie_mask)
+      for (int x = 0; x < W; ++x) tie_mask[size_t(y) * W + x] = uint8_t(tie[x]);
+    if (cand6) {
+      for (int x = 0; x < W; ++x) {
+        const size_t i = size_t(y) * W + x;
+        cand6[i] = cn[x] > 0 ? cs[2 * x] : kNone;
+        cand6[WH + i] = cn[x] > 1 ? cs[2 * x + 1] : kNone;
+        cand6[2 * WH + i] = cn[x] > 0 ? float(cfg.dmin + cd[2 * x]) : kNone;
+        cand6[3 * WH + i] = cn[x] > 1 ? float(cfg.dmin + cd[2 * x + 1]) : kNone;
+        cand6[5 * WH + i] = lrc2[x];
+      }
+    }
+  }
+  if (cand6) {
+    // Plane 4 keeps the -1e30 fill unless the reverse scan ran, then NaN where
+    // a pixel has no answer or its right pixel no claimant (de_dense.cpp:1581-1597).
+    for (size_t i = 0; i < WH; ++i) cand6[4 * WH + i] = cfg.lrc ? std::nanf("") : kNone;
+    if (cfg.lrc)
+      for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+          const size_t i = size_t(y) * W + x;
+          const float dl = disp[i];
+          if (!std::isfinite(dl)) continue;
+          const int xr = x - int(std::lround(dl));
+          if (xr >= 0 && xr < W && rd[size_t(y) * W + xr] >= 0)
+            cand6[4 * WH + i] = std::fabs(dl - float(cfg.dmin + rd[size_t(y) * W + xr]));
+        }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -457,59 +520,193 @@ int orc_dense_run(const uint8_t* L, const uint8_t* R, int W, int H,
       }
   }
 
-  const float q = 1.f / float(kOne);
-  std::vector<float> cs(size_t(W) * 2), cnb(size_t(W) * 2), lrc2(W);
-  std::vector<int> cd(size_t(W) * 2), cn(W), tie(W);
-  for (int y = 0; y < H; ++y) {
-    for (int x = 0; x < W; ++x) {
-      const size_t i = size_t(y) * W + x;
-      int n = 0;
-      cs[2 * x] = cs[2 * x + 1] = kNone;
-      cd[2 * x] = cd[2 * x + 1] = -1;
-      cnb[2 * x] = cnb[2 * x + 1] = kNone;
-      if (s0[i] > -32768) {
-        cs[2 * x] = float(s0[i]) * q; cd[2 * x] = d0[i]; n = 1;
-        if (cfg.subpixel) {
-          cnb[2 * x] = (m0[i] == -32768) ? kNone : float(m0[i]) * q;
-          cnb[2 * x + 1] = (p0[i] == -32768) ? kNone : float(p0[i]) * q;
-        }
-        if (s1[i] > -32768) { cs[2 * x + 1] = float(s1[i]) * q; cd[2 * x + 1] = d1[i]; n = 2; }
-      }
-      cn[x] = n;
-    }
-    solve_row_k2(W, cfg, cs.data(), cd.data(), cn.data(),
-                 cfg.subpixel ? cnb.data() : nullptr, disp + size_t(y) * W,
-                 margin + size_t(y) * W, cand6 ? lrc2.data() : nullptr,
-                 tie_mask ? tie.data() : nullptr);
-    if (tie_mask)
-      for (int x = 0; x < W; ++x) tie_mask[size_t(y) * W + x] = uint8_t(tie[x]);
-    if (cand6) {
-      for (int x = 0; x < W; ++x) {
-        const size_t i = size_t(y) * W + x;
-        cand6[i] = cn[x] > 0 ? cs[2 * x] : kNone;
-        cand6[WH + i] = cn[x] > 1 ? cs[2 * x + 1] : kNone;
-        cand6[2 * WH + i] = cn[x] > 0 ? float(cfg.dmin + cd[2 * x]) : kNone;
-        cand6[3 * WH + i] = cn[x] > 1 ? float(cfg.dmin + cd[2 * x + 1]) : kNone;
-        cand6[5 * WH + i] = lrc2[x];
-      }
-    }
-  }
-  if (cand6) {
-    // Plane 4 keeps the -1e30 fill unless the reverse scan ran, then NaN where
-    // a pixel has no answer or its right pixel no claimant (de_dense.cpp:1581-1597).
-    for (size_t i = 0; i < WH; ++i) cand6[4 * WH + i] = cfg.lrc ? std::nanf("") : kNone;
-    if (cfg.lrc)
-      for (int y = 0; y < H; ++y)
-        for (int x = 0; x < W; ++x) {
-          const size_t i = size_t(y) * W + x;
-          const float dl = disp[i];
-          if (!std::isfinite(dl)) continue;
-          const int xr = x - int(std::lround(dl));
-          if (xr >= 0 && xr < W && rd[size_t(y) * W + xr] >= 0)
-            cand6[4 * WH + i] = std::fabs(dl - float(cfg.dmin + rd[size_t(y) * W + xr]));
-        }
-  }
+  finish_dense(W, H, cfg, s0, s1, d0, d1, m0, p0, rd, disp, margin, cand6, tie_mask);
   return 0;
+}
+
+// --prior: the prior as a mask on absolute-disparity planes
+// (core/tools/de_dense.cpp:797-853, 1156-1296). Plane k is scored and
+// filtered on its rectangle only (the bounding box of the pixels whose band
+// |d - round(prior)| <= band admits it, padded by 8 and clipped to the score
+// region), the filter starting from the rectangle's own edge values; candidates
+// are inserted strictly inside each row's interval; no sub-pixel neighbours.
+// `prior` is W*H finite floats.
+int orc_dense_run_prior(const uint8_t* L, const uint8_t* R, int W, int H, const OrcCfg* cfgp,
+                        const float* prior, int band, float* disp, float* margin,
+                        float* cand6, uint8_t* tie_mask) {
+  if (!L || !R || !cfgp || !prior || !disp || !margin || W <= 0 || H <= 0) return -1;
+  OrcCfg cfg = *cfgp;
+  cfg.subpixel = 0;
+  const int hw = cfg.census_hw, hh = cfg.census_hh;
+  const int nb = (2 * hw + 1) * (2 * hh + 1) - 1;
+  if (nb > 64 || (nb & 1)) return -2;
+  const int D = cfg.dmax - cfg.dmin + 1;
+  if (D < 1 || D > 256) return -3;
+  const size_t WH = size_t(W) * H;
+  for (size_t i = 0; i < WH; ++i)
+    if (!std::isfinite(prior[i])) return -4;
+
+  std::vector<int16_t> mx0(size_t(H) * D, int16_t(W)), mx1(size_t(H) * D, int16_t(-1));
+  std::vector<int> pylo(D, H), pyhi(D, -1), pxlo(D, W), pxhi(D, -1);
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const int dc = int(std::lround(prior[size_t(y) * W + x]));
+      const int klo = std::max(cfg.dmin, dc - band) - cfg.dmin;
+      const int khi = std::min(cfg.dmax, dc + band) - cfg.dmin;
+      for (int k = klo; k <= khi; ++k) {
+        int16_t& a = mx0[size_t(y) * D + k];
+        int16_t& b = mx1[size_t(y) * D + k];
+        if (x < a) a = int16_t(x);
+        if (x > b) b = int16_t(x);
+      }
+    }
+  for (int y = 0; y < H; ++y)
+    for (int k = 0; k < D; ++k) {
+      const int a = mx0[size_t(y) * D + k], b = mx1[size_t(y) * D + k];
+      if (b < a) continue;
+      pylo[k] = std::min(pylo[k], y);
+      pyhi[k] = std::max(pyhi[k], y);
+      pxlo[k] = std::min(pxlo[k], a);
+      pxhi[k] = std::max(pxhi[k], b);
+    }
+
+  std::vector<uint64_t> cl(WH), cr(WH);
+  census_plane(L, W, H, hw, hh, cl.data());
+  census_plane(R, W, H, hw, hh, cr.data());
+  std::vector<uint16_t> ax(WH), ay(WH);
+  orc_rf_coeffs(L, W, H, cfg.sigma_s, cfg.sigma_r, ax.data(), ay.data());
+  const ScoreTables st = make_score_tables(nb / 2, cfg.ad, cfg.ad_trunc);
+
+  std::vector<int16_t> s0(WH, -32768), s1(WH, -32768), d0(WH, -1), d1(WH, -1);
+  std::vector<int16_t> m0(WH, -32768), p0(WH, -32768);
+  std::vector<int16_t> rs(cfg.lrc ? WH : 0, -32768), rd(cfg.lrc ? WH : 0, -1);
+  std::vector<int16_t> P(WH);
+  const int MPAD = 8;
+  for (int k = 0; k < D; ++k) {
+    const int d = cfg.dmin + k;
+    if (pyhi[k] < pylo[k]) continue;
+    const int y0 = std::max(hh, pylo[k] - MPAD), y1 = std::min(H - hh, pyhi[k] + 1 + MPAD);
+    const int x0 = std::max(hw + d, pxlo[k] - MPAD), x1 = std::min(W - hw, pxhi[k] + 1 + MPAD);
+    if (y1 - y0 < 1 || x1 - x0 < 2) continue;
+    for (int y = y0; y < y1; ++y)
+      for (int x = x0; x < x1; ++x) {
+        const size_t i = size_t(y) * W + x;
+        P[i] = graded(st, cl[i], cr[i - d], L[i], R[i - d]);
+      }
+    // rectangle filter (de_dense.cpp:434-483): H-fwd, H-bwd, V-fwd, V-bwd
+    for (int y = y0; y < y1; ++y) {
+      int16_t* r = &P[size_t(y) * W];
+      const uint16_t* a = &ax[size_t(y) * W];
+      int32_t f = r[x0];
+      for (int x = x0 + 1; x < x1; ++x) { f = rf_step(f, r[x], a[x]); r[x] = int16_t(f); }
+      f = r[x1 - 1];
+      for (int x = x1 - 2; x >= x0; --x) { f = rf_step(f, r[x], a[x + 1]); r[x] = int16_t(f); }
+    }
+    for (int y = y0 + 1; y < y1; ++y)
+      for (int x = x0; x < x1; ++x) {
+        const size_t i = size_t(y) * W + x;
+        P[i] = int16_t(rf_step(P[i - W], P[i], ay[i]));
+      }
+    for (int y = y1 - 2; y >= y0; --y)
+      for (int x = x0; x < x1; ++x) {
+        const size_t i = size_t(y) * W + x;
+        P[i] = int16_t(rf_step(P[i + W], P[i], ay[i + W]));
+      }
+    const int16_t kk = int16_t(k);
+    for (int y = std::max(y0, pylo[k]); y < std::min(y1, pyhi[k] + 1); ++y) {
+      const int xa = std::max(x0, int(mx0[size_t(y) * D + k]));
+      const int xb = std::min(x1, int(mx1[size_t(y) * D + k]) + 1);
+      for (int x = xa; x < xb; ++x) {
+        const size_t i = size_t(y) * W + x;
+        const int16_t v = P[i];
+        if (cfg.lrc) {
+          const size_t j = i - size_t(d);
+          if (v > rs[j]) { rs[j] = v; rd[j] = kk; }
+        }
+        if (v <= s1[i]) continue;
+        if (v > s0[i]) { s1[i] = s0[i]; d1[i] = d0[i]; s0[i] = v; d0[i] = kk; }
+        else { s1[i] = v; d1[i] = kk; }
+      }
+    }
+  }
+  finish_dense(W, H, cfg, s0, s1, d0, d1, m0, p0, rd, disp, margin, cand6, tie_mask);
+  return 0;
+}
+
+// Nearest upsample of a half-resolution prior (a disparity d at half scale is
+// 2d at full scale), de_dense.cpp:1789-1795 and 1958-1962.
+void orc_prior_upsample(const float* half, int W, int H, float* full) {
+  const int W2 = W / 2, H2 = H / 2;
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x)
+      full[size_t(y) * W + x] =
+          2.f * half[size_t(std::min(H2 - 1, y / 2)) * W2 + std::min(W2 - 1, x / 2)];
+}
+
+// --c2f (de_dense.cpp:1889-1966): a half-resolution pass with D/2 planes, one
+// iteration and min_margin 0; holes filled from row neighbours (forward, then
+// backward), then column neighbours; the upsampled result is the prior of the
+// masked full-resolution sweep. If holes remain, the full sweep runs instead.
+// `coarse` (nullable) receives the filled W/2 x H/2 coarse disparity.
+int orc_dense_run_c2f(const uint8_t* L, const uint8_t* R, int W, int H, const OrcCfg* cfgp,
+                      int band, float* disp, float* margin, float* cand6, uint8_t* tie_mask,
+                      float* coarse, int* used_mask) {
+  if (!L || !R || !cfgp || !disp || !margin || W < 2 || H < 2) return -1;
+  OrcCfg cfg = *cfgp;
+  cfg.subpixel = 0;
+  const int W2 = W / 2, H2 = H / 2;
+  std::vector<uint8_t> L2(size_t(W2) * H2), R2(size_t(W2) * H2);
+  for (int y = 0; y < H2; ++y)
+    for (int x = 0; x < W2; ++x) {
+      const size_t a = size_t(2 * y) * W + 2 * x;
+      L2[size_t(y) * W2 + x] = uint8_t((int(L[a]) + L[a + 1] + L[a + W] + L[a + W + 1] + 2) / 4);
+      R2[size_t(y) * W2 + x] = uint8_t((int(R[a]) + R[a + 1] + R[a + W] + R[a + W + 1] + 2) / 4);
+    }
+  OrcCfg cc = cfg;
+  cc.dmin = std::max(1, cfg.dmin / 2);
+  cc.dmax = (cfg.dmax + 1) / 2;
+  cc.min_margin = 0.f;
+  cc.iters = 1;
+  std::vector<float> dh(size_t(W2) * H2), mh(size_t(W2) * H2);
+  if (int rc = orc_dense_run(L2.data(), R2.data(), W2, H2, &cc, dh.data(), mh.data(), nullptr,
+                             nullptr))
+    return rc;
+  for (int y = 0; y < H2; ++y) {
+    float* r = &dh[size_t(y) * W2];
+    float last = std::nanf("");
+    for (int x = 0; x < W2; ++x) {
+      if (std::isfinite(r[x])) last = r[x];
+      else if (std::isfinite(last)) r[x] = last;
+    }
+    last = std::nanf("");
+    for (int x = W2 - 1; x >= 0; --x) {
+      if (std::isfinite(r[x])) last = r[x];
+      else if (std::isfinite(last)) r[x] = last;
+    }
+  }
+  for (int x = 0; x < W2; ++x) {
+    float last = std::nanf("");
+    for (int y = 0; y < H2; ++y) {
+      float& v = dh[size_t(y) * W2 + x];
+      if (std::isfinite(v)) last = v;
+      else if (std::isfinite(last)) v = last;
+    }
+    last = std::nanf("");
+    for (int y = H2 - 1; y >= 0; --y) {
+      float& v = dh[size_t(y) * W2 + x];
+      if (std::isfinite(v)) last = v;
+      else if (std::isfinite(last)) v = last;
+    }
+  }
+  if (coarse) std::copy(dh.begin(), dh.end(), coarse);
+  size_t holes = 0;
+  for (float v : dh) holes += std::isfinite(v) ? 0 : 1;
+  if (used_mask) *used_mask = holes ? 0 : 1;
+  if (holes) return orc_dense_run(L, R, W, H, &cfg, disp, margin, cand6, tie_mask);
+  std::vector<float> prior(size_t(W) * H);
+  orc_prior_upsample(dh.data(), W, H, prior.data());
+  return orc_dense_run_prior(L, R, W, H, &cfg, prior.data(), band, disp, margin, cand6,
+                             tie_mask);
 }
 
 // Aggregated FP32 volume of the full-volume path (--no-blockwise / --dump-vol):

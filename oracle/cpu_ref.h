@@ -39,6 +39,18 @@ int orc_volume_f32(const uint8_t* L, const uint8_t* R, int W, int H,
 float orc_confidence(float s1, float s2, float lambda, float lrc, float lrc_tol);
 uint8_t orc_confidence_u8(float s1, float s2, float lambda, float lrc, float lrc_tol);
 
+/* --prior / --c2f (core/tools/de_dense.cpp:797-853, 1156-1296, 1889-1966):
+ * the prior as a mask on absolute-disparity planes; sub-pixel off. prior is
+ * W*H finite floats. c2f: coarse (nullable) gets the filled W/2 x H/2 coarse
+ * disparity, *used_mask 1 when the masked sweep ran (0: holes, full sweep). */
+int orc_dense_run_prior(const uint8_t* L, const uint8_t* R, int W, int H, const OrcCfg* cfg,
+                        const float* prior, int band, float* disp, float* margin,
+                        float* cand6, uint8_t* tie_mask);
+void orc_prior_upsample(const float* half, int W, int H, float* full);
+int orc_dense_run_c2f(const uint8_t* L, const uint8_t* R, int W, int H, const OrcCfg* cfg,
+                      int band, float* disp, float* margin, float* cand6, uint8_t* tie_mask,
+                      float* coarse, int* used_mask);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,5 +43,7 @@ def flags_to_kwargs(flags: str) -> dict:
         elif a == "--no-subpixel": kw["subpixel"] = 0; i += 1
         elif a == "--fit-parabola": kw["fit_eq"] = 0; i += 1
         elif a == "--lrc": kw["lrc"] = 1; i += 1
+        elif a == "--band": kw["band"] = int(t[i + 1]); i += 2
+        elif a == "--c2f": kw["c2f"] = 1; i += 1
         else: raise ValueError(a)
     return kw
