@@ -654,6 +654,71 @@ def bench_pipe(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_c2f(a):
+    """The prior as a mask: one 848x480 slanted pair at D = 128 (d 0..127),
+    8 iterations, through the full sweep, `--prior` (a full-resolution prior
+    from the pair's ground truth, band 2) and `--c2f`, host images to host
+    disparity. The CPU baseline is the reference `de_dense --c2f --threads 1`
+    (oracle/_ref/de_dense) on the same pair, one run."""
+    import numpy as np
+
+    from doubleeye_amd import synth
+    from doubleeye_amd.dense import Cfg, Context
+
+    ctx = Context(0)
+    L, R, gt = synth.slanted_pair(848, 480, seed=2)[:3]
+    cfg = Cfg(dmax=127, iters=8, subpixel=0)
+    prior = np.nan_to_num(gt, nan=float(np.nanmedian(gt))).astype(np.float32)
+    reps = max(5, a.steps // 20)
+
+    def timed(fn):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        return (time.perf_counter() - t0) * 1e3 / reps, out
+
+    ms_full, full = timed(lambda: ctx.run_dense(L, R, cfg))
+    ms_prior, pri = timed(lambda: ctx.run_dense_prior(L, R, prior, cfg, 2))
+    ms_c2f, c2f = timed(lambda: ctx.run_dense_c2f(L, R, cfg, 2))
+
+    def bad1(d):
+        ok = np.isfinite(d) & np.isfinite(gt)
+        return round(float((np.abs(d[ok] - gt[ok]) > 1.0).mean()), 4), round(float(ok.mean()), 4)
+
+    out = {
+        "metric": "dense 848x480 D=128 host-to-host time: full sweep vs --prior vs --c2f",
+        "value": round(ms_c2f, 3), "unit": "ms", "higher_is_better": False,
+        "full_ms": round(ms_full, 3), "prior_ms": round(ms_prior, 3), "c2f_ms": round(ms_c2f, 3),
+        "c2f_used_mask": bool(c2f[4]),
+        "bad1_coverage": {"full": bad1(full[0]), "prior": bad1(pri[0]), "c2f": bad1(c2f[0])},
+        "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": round(ms_c2f, 3),
+        "scaling": "weak", "vs_baseline": None, "dtype": "int16 volume / f32 messages",
+        "data": "synthetic (slanted planes 848x480, seed 2)",
+        "config": {"workload": "848x480, d 0..127, 8 iterations, band 2, no sub-pixel"},
+    }
+    if not a.no_cpu_baseline:
+        exe = os.path.join(ROOT, "oracle", "_ref", "de_dense")
+        if os.path.exists(exe):
+            with tempfile.TemporaryDirectory() as td:
+                lp, rp, op = (os.path.join(td, n) for n in ("l.y8", "r.y8", "o.f32"))
+                L.tofile(lp)
+                R.tofile(rp)
+                cmd = [exe, lp, rp, "848", "480", "--threads", "1", "--dmax", "127", "--iters",
+                       "8", "--no-subpixel", "--c2f", "--out", op]
+                t0 = time.perf_counter()
+                subprocess.run(cmd, check=True, capture_output=True)
+                t = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(t * 1e3, 1), "unit": "ms", "cores": 1,
+                                   "kind": "reference",
+                                   "sample": "one run of de_dense --c2f --threads 1 (process "
+                                             "wall time, file I/O included)"}
+        else:
+            out["cpu_baseline"] = {"error": "oracle/_ref/de_dense not built on this host"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -663,7 +728,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--workload", default="c2",
                     choices=["c2", "c3", "c4", "c5", "sparse", "dense_masda", "detect",
-                             "pipe"],
+                             "pipe", "c2f"],
                     help="c2: one 848x480x128 pair (default, the headline); c4: 64-pair "
                          "batch; c3: keypoint-graph BP; c5: serving loop; sparse: keypoint "
                          "match_masda; detect: keypoint detector; pipe: de_pipe's pair")
@@ -697,6 +762,9 @@ def main():
         return
     if a.workload == "pipe":
         bench_pipe(a)
+        return
+    if a.workload == "c2f":
+        bench_c2f(a)
         return
 
     import numpy as np

@@ -633,46 +633,74 @@ __global__ __launch_bounds__(256) void k_mask_dc(const float* __restrict__ prior
   dc[i] = r >= 1e9f ? 1000000000 : (r <= -1e9f ? -1000000000 : int(r));
 }
 
-// One thread per (row, plane): the row's interval [mx0, mx1] of pixels whose
-// band admits the plane (mx0 = W, mx1 = -1 when none).
+// One workgroup per row: the interval [mx0, mx1] of the row's pixels whose
+// band admits each plane (mx0 = W, mx1 = -1 when none), by LDS min/max.
 __global__ __launch_bounds__(256) void k_mask_iv(const FrameDesc* __restrict__ fp,
                                                  const int* __restrict__ dc, int band,
                                                  int* __restrict__ mx0, int* __restrict__ mx1) {
   const FrameDesc F = *fp;
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= F.H * F.D) return;
-  const int y = t / F.D, k = t - y * F.D;
-  const long long d = F.dmin + k;
+  __shared__ int lo[kMaxD], hi[kMaxD];
+  const int y = blockIdx.x;
+  for (int k = threadIdx.x; k < F.D; k += 256) {
+    lo[k] = F.W;
+    hi[k] = -1;
+  }
+  __syncthreads();
   const int* row = dc + size_t(y) * F.W;
-  int a = F.W, b = -1;
-  for (int x = 0; x < F.W; ++x) {
+  for (int x = threadIdx.x; x < F.W; x += 256) {
     const long long c = row[x];
-    if (d >= c - band && d <= c + band) {
-      a = min(a, x);
-      b = x;
+    const long long k0 = max(c - band, (long long)F.dmin) - F.dmin;
+    const long long k1 = min(c + band, (long long)(F.dmin + F.D - 1)) - F.dmin;
+    for (long long k = k0; k <= k1; ++k) {
+      atomicMin(&lo[k], x);
+      atomicMax(&hi[k], x);
     }
   }
-  mx0[t] = a;
-  mx1[t] = b;
+  __syncthreads();
+  for (int k = threadIdx.x; k < F.D; k += 256) {
+    mx0[size_t(y) * F.D + k] = lo[k];
+    mx1[size_t(y) * F.D + k] = hi[k];
+  }
 }
 
-// One thread per plane: the bounding box of its intervals and the filtered
+// One workgroup per plane: the bounding box of its intervals and the filtered
 // rectangle (y0, y1, x0, x1 half-open; y1 = y0 = 0 when the plane is skipped).
 __global__ __launch_bounds__(256) void k_mask_rect(const FrameDesc* __restrict__ fp,
                                                    const int* __restrict__ mx0,
                                                    const int* __restrict__ mx1,
                                                    int4* __restrict__ box, int4* __restrict__ rect) {
   const FrameDesc F = *fp;
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= F.D) return;
+  const int k = blockIdx.x;
   int ylo = F.H, yhi = -1, xlo = F.W, xhi = -1;
-  for (int y = 0; y < F.H; ++y) {
+  for (int y = threadIdx.x; y < F.H; y += 256) {
     const int a = mx0[size_t(y) * F.D + k], b = mx1[size_t(y) * F.D + k];
     if (b < a) continue;
     ylo = min(ylo, y);
-    yhi = y;
+    yhi = max(yhi, y);
     xlo = min(xlo, a);
     xhi = max(xhi, b);
+  }
+  __shared__ int red[4][256 / kWave];
+  for (int o = 1; o < kWave; o <<= 1) {
+    ylo = min(ylo, __shfl_xor(ylo, o));
+    yhi = max(yhi, __shfl_xor(yhi, o));
+    xlo = min(xlo, __shfl_xor(xlo, o));
+    xhi = max(xhi, __shfl_xor(xhi, o));
+  }
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    red[0][w] = ylo;
+    red[1][w] = yhi;
+    red[2][w] = xlo;
+    red[3][w] = xhi;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int q = 1; q < 256 / kWave; ++q) {
+    ylo = min(ylo, red[0][q]);
+    yhi = max(yhi, red[1][q]);
+    xlo = min(xlo, red[2][q]);
+    xhi = max(xhi, red[3][q]);
   }
   constexpr int kPad = 8;   // MPAD: real scores this far outside the intervals
   const int d = F.dmin + k;
@@ -729,22 +757,52 @@ __global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
   const int sh = F.sc_shift + 1;
   const uint32_t T2 = 2u * uint32_t(F.ad_T);
   uint16_t* const x0p = F.vol + size_t(y) * W * Dp + g * kWave;
+  constexpr int kU = 16;   // scores (forward) and loads (backward) in flight
   uint32_t t2 = 0u;
-  for (int x = xmin; x < xmax; ++x) {
-    const uint4 rec = lr[x];
-    if (x >= xs && x < xe) {
-      const uint32_t c = score_at(smem, crs, r2s, rec, x - d, sh, T2);
-      t2 = x == xs ? (c << 16) : rf_step_rec(rec, t2, c);
-      x0p[size_t(x) * Dp + lane] = hi16(t2);
+  for (int xb = xmin; xb < xmax; xb += kU) {
+    uint32_t c[kU];
+    uint4 rec[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int x = xb + u;
+      rec[u] = lr[min(x, W - 1)];
+      c[u] = 0u;
+      if (x < xe && x >= xs) c[u] = score_at(smem, crs, r2s, rec[u], x - d, sh, T2);
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int x = xb + u;
+      if (x < xe && x >= xs) {
+        t2 = x == xs ? (c[u] << 16) : rf_step_rec(rec[u], t2, c[u]);
+        x0p[size_t(x) * Dp + lane] = hi16(t2);
+      }
     }
   }
-  for (int x = xmax - 2; x >= xmin; --x) {
-    const uint4 rec = lr[x + 1];
-    if (x >= xs && x < xe - 1) {
-      const uint32_t c = x0p[size_t(x) * Dp + lane];
-      t2 = rf_step_rec(rec, t2, c);
-      x0p[size_t(x) * Dp + lane] = hi16(t2);
+  // backward: the forward values of the next chunk are loaded while the
+  // current chunk runs its chain (the chunks touch disjoint columns)
+  auto load_bwd = [&](int xt, uint32_t* v) {
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int x = xt - u;
+      v[u] = 0u;
+      if (x >= xs && x < xe - 1) v[u] = x0p[size_t(x) * Dp + lane];
     }
+  };
+  uint32_t cur[kU], nxt[kU];
+  load_bwd(xmax - 2, cur);
+  for (int xt = xmax - 2; xt >= xmin; xt -= kU) {
+    if (xt - kU >= xmin) load_bwd(xt - kU, nxt);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int x = xt - u;
+      const uint4 rec = lr[max(x + 1, 0)];
+      if (x >= xs && x < xe - 1) {
+        t2 = rf_step_rec(rec, t2, cur[u]);
+        x0p[size_t(x) * Dp + lane] = hi16(t2);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) cur[u] = nxt[u];
   }
 }
 
@@ -765,25 +823,56 @@ __global__ __launch_bounds__(64) void k_mcol(const FrameDesc* __restrict__ fp,
   const size_t rs = size_t(W) * Dp;
   uint16_t* const col = F.vol + size_t(x) * Dp + g * kWave + lane;
   const uint32_t* ay = F.ayt + size_t(x) * H;
+  constexpr int kU = 8;   // rows in flight
+  auto load_rows = [&](int y0, int step, int lo, int hi, uint32_t* v, uint32_t* a2v, int aoff) {
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int y = y0 + step * u;
+      a2v[u] = ay[min(max(y + aoff, 0), H - 1)];
+      v[u] = 0u;
+      if (y >= lo && y < hi) v[u] = col[size_t(y) * rs];
+    }
+  };
   uint32_t t = 0u;
-  for (int y = ymin; y < ymax; ++y) {
-    const uint32_t a2 = ay[y];
-    if (y >= ys && y < ye) {
-      const uint32_t c = col[size_t(y) * rs];
-      if (y == ys) {
-        t = c << 16;
-      } else {
-        t = rf_chain_v(a2, t, rf_acc(a2, c));
+  uint32_t cur[kU], nxt[kU], ca[kU], na[kU];
+  // V-fwd, rows ascending; the next chunk's rows load during this chunk's chain
+  load_rows(ymin, 1, ys, ye, cur, ca, 0);
+  for (int yb = ymin; yb < ymax; yb += kU) {
+    if (yb + kU < ymax) load_rows(yb + kU, 1, ys, ye, nxt, na, 0);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int y = yb + u;
+      if (y >= ys && y < ye) {
+        if (y == ys) {
+          t = cur[u] << 16;
+        } else {
+          t = rf_chain_v(ca[u], t, rf_acc(ca[u], cur[u]));
+          col[size_t(y) * rs] = hi16(t);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      cur[u] = nxt[u];
+      ca[u] = na[u];
+    }
+  }
+  // V-bwd, rows descending, coefficient of y + 1
+  load_rows(ymax - 2, -1, ys, ye - 1, cur, ca, 1);
+  for (int yt = ymax - 2; yt >= ymin; yt -= kU) {
+    if (yt - kU >= ymin) load_rows(yt - kU, -1, ys, ye - 1, nxt, na, 1);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int y = yt - u;
+      if (y >= ys && y < ye - 1) {
+        t = rf_chain_v(ca[u], t, rf_acc(ca[u], cur[u]));
         col[size_t(y) * rs] = hi16(t);
       }
     }
-  }
-  for (int y = ymax - 2; y >= ymin; --y) {
-    const uint32_t a2 = ay[y + 1];
-    if (y >= ys && y < ye - 1) {
-      const uint32_t c = col[size_t(y) * rs];
-      t = rf_chain_v(a2, t, rf_acc(a2, c));
-      col[size_t(y) * rs] = hi16(t);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      cur[u] = nxt[u];
+      ca[u] = na[u];
     }
   }
 }
@@ -842,12 +931,10 @@ __global__ __launch_bounds__(256) void k_mlrc(const FrameDesc* __restrict__ fp,
 
 hipError_t launch_masked(const FrameDesc& hf, const FrameDesc* df, const float* prior, int band,
                          const MaskBufs& m, size_t lds_row, bool lrc, hipStream_t s) {
-  const int WH = hf.W * hf.H, HD = hf.H * hf.D;
+  const int WH = hf.W * hf.H;
   hipLaunchKernelGGL(k_mask_dc, dim3((WH + 255) / 256), dim3(256), 0, s, prior, WH, m.dc);
-  hipLaunchKernelGGL(k_mask_iv, dim3((HD + 255) / 256), dim3(256), 0, s, df, m.dc, band, m.mx0,
-                     m.mx1);
-  hipLaunchKernelGGL(k_mask_rect, dim3((hf.D + 255) / 256), dim3(256), 0, s, df, m.mx0, m.mx1,
-                     m.box, m.rect);
+  hipLaunchKernelGGL(k_mask_iv, dim3(hf.H), dim3(256), 0, s, df, m.dc, band, m.mx0, m.mx1);
+  hipLaunchKernelGGL(k_mask_rect, dim3(hf.D), dim3(256), 0, s, df, m.mx0, m.mx1, m.box, m.rect);
   hipLaunchKernelGGL(k_mrow, dim3(hf.H * hf.G), dim3(64), lds_row, s, df, m.rect);
   hipLaunchKernelGGL(k_mcol, dim3(hf.W * hf.G), dim3(64), 0, s, df, m.rect);
   hipLaunchKernelGGL(k_minsert, dim3((WH + 255) / 256), dim3(256), 0, s, df, m.rect, m.box,

@@ -36,43 +36,56 @@ __global__ __launch_bounds__(256) void k_half(const uint8_t* __restrict__ L,
   R2[i] = uint8_t((int(R[a]) + R[a + 1] + R[a + W] + R[a + W + 1] + 2) / 4);
 }
 
-// Nearest-neighbour hole fill along each row: forward carry, then backward.
-__global__ __launch_bounds__(256) void k_fill_rows(float* __restrict__ d, int W, int H) {
-  const int y = blockIdx.x * 256 + threadIdx.x;
-  if (y >= H) return;
-  float* r = d + size_t(y) * W;
-  float last = __int_as_float(0x7fc00000);
-  for (int x = 0; x < W; ++x) {
-    if (isfinite(r[x])) last = r[x];
-    else if (isfinite(last)) r[x] = last;
+// Nearest-neighbour hole fill of n values at stride `st`, one wave per line:
+// the reference's forward carry then backward carry leaves every hole with the
+// last finite value before it, or, for the holes before the first finite value,
+// that first value. The forward carry is a prefix max of finite indices;
+// finite values are only read and holes only written, so the line is filled in
+// place. Returns (through the wave) the holes left.
+__device__ int fill_line(float* d, int n, size_t st) {
+  const int lane = __lane_id();
+  int first = n;
+  for (int b = 0; b < n; b += kWave) {
+    const int i = b + lane;
+    const bool f = i < n && isfinite(d[size_t(i) * st]);
+    const uint64_t m = __ballot(f);
+    if (m) {
+      first = b + __ffsll(static_cast<unsigned long long>(m)) - 1;
+      break;
+    }
   }
-  last = __int_as_float(0x7fc00000);
-  for (int x = W - 1; x >= 0; --x) {
-    if (isfinite(r[x])) last = r[x];
-    else if (isfinite(last)) r[x] = last;
+  if (first == n) return n;   // nothing to carry: all holes stay
+  const float f0 = d[size_t(first) * st];
+  int carry = -1;
+  for (int b = 0; b < n; b += kWave) {
+    const int i = b + lane;
+    const bool in = i < n;
+    const float v = in ? d[size_t(i) * st] : 0.f;
+    const bool fin = in && isfinite(v);
+    int idx = fin ? i : -1;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int t = __shfl_up(idx, o);
+      if (lane >= o) idx = max(idx, t);
+    }
+    idx = max(idx, carry);
+    if (in && !fin) d[size_t(i) * st] = idx >= 0 ? d[size_t(idx) * st] : f0;
+    carry = __shfl(idx, kWave - 1);
   }
+  return 0;
 }
 
-// The same down each column, counting what is still a hole.
+__global__ __launch_bounds__(256) void k_fill_rows(float* __restrict__ d, int W, int H) {
+  const int y = blockIdx.x * (256 / kWave) + int(threadIdx.x) / kWave;
+  if (y >= H) return;   // uniform over the wave
+  (void)fill_line(d + size_t(y) * W, W, 1);
+}
+
 __global__ __launch_bounds__(256) void k_fill_cols(float* __restrict__ d, int W, int H,
                                                    int* __restrict__ holes) {
-  const int x = blockIdx.x * 256 + threadIdx.x;
-  if (x >= W) return;
-  float last = __int_as_float(0x7fc00000);
-  for (int y = 0; y < H; ++y) {
-    float& v = d[size_t(y) * W + x];
-    if (isfinite(v)) last = v;
-    else if (isfinite(last)) v = last;
-  }
-  last = __int_as_float(0x7fc00000);
-  int n = 0;
-  for (int y = H - 1; y >= 0; --y) {
-    float& v = d[size_t(y) * W + x];
-    if (isfinite(v)) last = v;
-    else if (isfinite(last)) v = last;
-    n += isfinite(v) ? 0 : 1;
-  }
-  if (n) atomicAdd(holes, n);
+  const int x = blockIdx.x * (256 / kWave) + int(threadIdx.x) / kWave;
+  if (x >= W) return;   // uniform over the wave
+  const int left = fill_line(d + x, H, size_t(W));
+  if (left && __lane_id() == 0) atomicAdd(holes, left);
 }
 
 // Nearest upsample of a half-resolution disparity map, doubled.
@@ -181,8 +194,10 @@ static int c2f_dev(Ctx& x, const uint8_t* dL, const uint8_t* dR, int W, int H, d
   DE_HIP(hipGetLastError());
   if (int rc = x.dense_dev(L2, R2, W2, H2, cc, dh, mh, nullptr, s, 0xF)) return rc;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, holes);
-  hipLaunchKernelGGL(k_fill_rows, dim3((H2 + 255) / 256), dim3(256), 0, s, dh, W2, H2);
-  hipLaunchKernelGGL(k_fill_cols, dim3((W2 + 255) / 256), dim3(256), 0, s, dh, W2, H2, holes);
+  constexpr int kLines = 256 / kWave;   // lines per workgroup
+  hipLaunchKernelGGL(k_fill_rows, dim3((H2 + kLines - 1) / kLines), dim3(256), 0, s, dh, W2, H2);
+  hipLaunchKernelGGL(k_fill_cols, dim3((W2 + kLines - 1) / kLines), dim3(256), 0, s, dh, W2, H2,
+                     holes);
   DE_HIP(hipGetLastError());
   if (coarse) *coarse = dh;
   int nh = 0;
