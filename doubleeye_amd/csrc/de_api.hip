@@ -170,6 +170,7 @@ int Ctx::frame_desc(const de_cfg& c, int W, int H, const uint8_t* L, const uint8
   const int D = c.dmax - c.dmin + 1;
   fd->L = L; fd->R = R;
   fd->lrec = sp.lrec; fd->cr = sp.cr; fd->ayt = sp.ayt; fd->vol = sp.vol; fd->cand = sp.cand;
+  fd->fwd = sp.fwd;
   fd->disp = disp; fd->margin = margin; fd->cand6 = cand6; fd->conf = conf; fd->q4 = q4;
   fd->rf = rf; fd->sc = sc;
   fd->W = W; fd->H = H; fd->D = D; fd->G = (D + kWave - 1) / kWave; fd->Dpad = fd->G * kWave;
@@ -189,7 +190,8 @@ size_t scratch_bytes(int W, int H, int D) {
   const size_t WH = size_t(W) * H;
   const int Dpad = ((D + kWave - 1) / kWave) * kWave;
   const size_t vol_rows = size_t(H) + 2 * kVolPadRows;
-  return WH * (16 + 8 + 4 + 16 + 2) + vol_rows * W * size_t(Dpad) * 2 + 6 * 256;
+  const size_t fwd = size_t(H) * (Dpad / kWave) * ((W + 7) / 8) * 1024;
+  return WH * (16 + 8 + 4 + 16 + 2) + vol_rows * W * size_t(Dpad) * 2 + fwd + 7 * 256;
 }
 
 // Carve one frame's scratch out of a device block.
@@ -208,6 +210,8 @@ Scratch carve_scratch(void* base, int W, int H, int D) {
   const size_t row_elems = size_t(W) * Dpad;
   uint16_t* v = reinterpret_cast<uint16_t*>(take((size_t(H) + 2 * kVolPadRows) * row_elems * 2));
   s.vol = v + kVolPadRows * row_elems;
+  // row-pass forward values: per (row, group), ceil(W/8) blocks of 1 KB
+  s.fwd = reinterpret_cast<uint16_t*>(take(size_t(H) * (Dpad / kWave) * ((W + 7) / 8) * 1024));
   return s;
 }
 
@@ -266,22 +270,32 @@ int Ctx::launch_stages(const Prepared& P, const BatchView& bv, hipStream_t s, in
   if ((stage_mask & 1) && (e = launch_prep(bv, items(kStagePrep), fds[0].hw, fds[0].hh, s)) != hipSuccess)
     return hip_fail(e, "k_prep");
   if (ev) DE_HIP(hipEventRecord(ev->e[1], s));
-  if ((stage_mask & 2) && (e = launch_row(bv, items(kStageRow), P.lds_row, P.rowv, s)) != hipSuccess)
-    return hip_fail(e, "k_row");
-  if (ev) DE_HIP(hipEventRecord(ev->e[2], s));
-  if (stage_mask & 4) {
+  // row and column passes: one launch per disparity-group count (frames are
+  // sorted by G)
+  auto per_g = [&](int st, auto&& launch) -> hipError_t {
     for (int i = 0; i < n;) {
       int j = i;
       while (j < n && fds[j].G == fds[i].G) ++j;
-      const int base = P.start[size_t(kStageCol) * (n + 1) + i];
-      const int cnt = P.start[size_t(kStageCol) * (n + 1) + j] - base;
-      bool store = false;
-      for (int q = i; q < j; ++q) store = store || fds[q].store_vol != 0;
-      if ((e = launch_col(bv, base, cnt, fds[i].G, store, P.colv, s)) != hipSuccess)
-        return hip_fail(e, "k_col");
+      const int base = P.start[size_t(st) * (n + 1) + i];
+      const int cnt = P.start[size_t(st) * (n + 1) + j] - base;
+      if (hipError_t r = launch(i, j, base, cnt)) return r;
       i = j;
     }
-  }
+    return hipSuccess;
+  };
+  if ((stage_mask & 2) &&
+      (e = per_g(kStageRow, [&](int i, int, int base, int cnt) {
+         return launch_row(bv, base, cnt, fds[i].G, P.lds_row, P.rowv, s);
+       })) != hipSuccess)
+    return hip_fail(e, "k_row");
+  if (ev) DE_HIP(hipEventRecord(ev->e[2], s));
+  if ((stage_mask & 4) &&
+      (e = per_g(kStageCol, [&](int i, int j, int base, int cnt) {
+         bool store = false;
+         for (int q = i; q < j; ++q) store = store || fds[q].store_vol != 0;
+         return launch_col(bv, base, cnt, fds[i].G, store, P.colv, s);
+       })) != hipSuccess)
+    return hip_fail(e, "k_col");
   bool any_lrc = false;
   for (int q = 0; q < n; ++q) any_lrc = any_lrc || fds[q].lrc != 0;
   if ((stage_mask & 4) && any_lrc &&

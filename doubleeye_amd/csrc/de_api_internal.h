@@ -40,6 +40,7 @@ struct Scratch {
   uint32_t* ayt;
   uint32_t* cand;
   uint16_t* vol;
+  uint16_t* fwd;
   int16_t* rdisp;
 };
 size_t scratch_bytes(int W, int H, int D);
@@ -116,7 +117,8 @@ struct Ctx {
 size_t row_lds_bytes(int W, int nbits, int sc_shift);
 size_t solve_lds_bytes(int W);
 hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_t s);
-hipError_t launch_row(const BatchView& b, int items, size_t lds, int variant, hipStream_t s);
+hipError_t launch_row(const BatchView& b, int base, int items, int G, size_t lds, int variant,
+                      hipStream_t s);
 hipError_t launch_col(const BatchView& b, int base, int items, int G, bool store, int variant,
                       hipStream_t s);
 hipError_t launch_solve(const BatchView& b, int items, size_t lds, hipStream_t s);

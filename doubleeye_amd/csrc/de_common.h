@@ -54,6 +54,8 @@ struct FrameDesc {
   uint64_t* cr;          // right census plane, W*H
   uint32_t* ayt;         // 2*ay, transposed [x][y] (scalar loads down a column)
   uint16_t* vol;         // aggregated volume v', [y][x][Dpad]
+  uint16_t* fwd;         // row-pass forward values, per (row, group) ceil(W/8) blocks of
+                         //   [64 lanes][8 columns] u16 (1 KB)
   uint32_t* cand;        // per pixel 4 words: key0, key1, (nbm | nbp << 16), 0
   int16_t* rdisp;        // exact reverse match per right pixel (lrc only), W*H
   float* disp;           // outputs, W*H (NaN = no answer)

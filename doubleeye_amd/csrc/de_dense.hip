@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "de_batch.h"
 #include "de_common.h"
 
@@ -146,147 +148,373 @@ __device__ __forceinline__ uint32_t rf_step_rec(const uint4& rec, uint32_t t2, u
   return rf_chain_v(a2, t2, __umul24(rec.w, c) + 32768u);
 }
 
-// U: H-bwd prefetch depth (columns of forward values in flight per lane)
-template <int U>
-__global__ __launch_bounds__(64) void k_row(BatchView b) {
+// The launch has about one wave per SIMD (one wave per row and group), so the
+// loops are written to hide latency inside the wave and to a budget per
+// (x, d) step. Forward: 10 VALU (2 xor, 2 bcnt, mask, sad, min, table index,
+// filter accumulator, chain), half a pack, 3 LDS reads, an eighth of a store.
+// Backward: 3 VALU (unpack, accumulator, chain), one LDS write, half an LDS
+// read, an eighth of a load and of a store.
+//
+// The forward values go to a scratch private to the (row, group) wave, in
+// blocks of 8 columns laid out [x / 8][lane][x % 8]: a lane's 8 values are 16
+// contiguous bytes, so the forward stores and the backward loads one 16-byte
+// word per lane per 8 columns (one memory instruction where a column-major
+// layout needs eight, which keeps deep prefetch within the vector memory
+// counter). The backward pass's final values are transposed through a 1 KB
+// LDS tile and leave as 16-byte stores into the [y][x][Dpad] volume, lane l
+// writing disparities 8(l%8)..8(l%8)+7 of column 8b + l/8.
+//
+// The forward scan is software-pipelined over 8-column blocks in three
+// stages: LDS loads of block n+2, score-table reads of block n+1, chain of
+// block n, so neither LDS latency is waited on where it is issued.
+//
+// The record word z = 2*L | 2a << 16 feeds the chain through op_sel (high
+// half); the intensity difference masks the low half first (v_sad_u16 would
+// add the high halves' difference too: it sums both halves).
+
+// 2*f' = hi16(z) * hi16(t2prev) + acc, with acc = (65536 - 2a) * c' + 32768
+// computed off the dependency chain.
+__device__ __forceinline__ uint32_t rf_chain_z(uint32_t z, uint32_t t2prev, uint32_t acc) {
+  uint32_t r;
+  asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[1,1,0,0]" : "=v"(r) : "v"(z), "v"(t2prev), "v"(acc));
+  return r;
+}
+
+// popcount(a) + acc in one instruction (kept as written: the compiler would
+// otherwise sum two counts and shift in a fused add-shift plus another add)
+__device__ __forceinline__ uint32_t bcnt_acc(uint32_t a, uint32_t acc) {
+  uint32_t r;
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(acc));
+  return r;
+}
+
+struct RowLds {
+  const uint4* lr;           // left records of the row (z low half = kTblRow * L)
+  const uint64_t* crs;       // right census of the row
+  const uint32_t* r2s;       // kTblRow * R of the row (a word each)
+  uint32_t Tb;               // kTblRow * T
+};
+
+// Score table as staged for the row pass: intensity-difference-major, rows of
+// kTblRow bytes (66 u16, the census distance 0..62 in a row). A wave's 64
+// lanes share the intensity row when their differences saturate at T (the
+// common case), and the census distance then spreads them over the banks; the
+// padded stride (33 words) also separates lanes of equal distance on
+// different rows. Census-distance-major rows of 2^s entries put every
+// saturated lane on one bank of each row.
+constexpr uint32_t kTblRow = 132;
+
+// Score-table read; the table sits at LDS address 0 (k_row declares no static
+// LDS, so its dynamic LDS starts there) and the byte offset kTblRow*min(|L - R|,
+// T) + 2h is the address itself.
+__device__ __forceinline__ uint32_t row_tbl(uint32_t h, uint32_t adb) {
+  return *reinterpret_cast<const __attribute__((address_space(3))) uint16_t*>(
+      uintptr_t((h << 1) + adb));
+}
+
+__device__ __forceinline__ uint32_t row_score(const RowLds& q, const uint4& rec, uint64_t cr,
+                                              uint32_t rb) {
+  const uint32_t h = bcnt_acc(rec.y ^ uint32_t(cr >> 32), __popc(rec.x ^ uint32_t(cr)));
+  const uint32_t adb = min(sad_u32(rec.z & 0xFFFFu, rb, 0u), q.Tb);
+  return row_tbl(h, adb);
+}
+
+// LDS operands of 8 columns (stage 1) and their scores with the record words
+// the chain needs (stage 2; the table reads may still be in flight).
+struct RowRaw {
+  uint4 rec[8];
+  uint64_t cr[8];
+  uint32_t r2[8];
+};
+struct RowBlk {
+  uint32_t c[8];
+  uint32_t z[8];
+  uint32_t w[8];
+};
+
+__device__ __forceinline__ void row_load_raw(const RowLds& q, RowRaw& R, int x0, int d) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    R.rec[u] = q.lr[x0 + u];
+    R.cr[u] = q.crs[x0 + u - d];
+    R.r2[u] = q.r2s[x0 + u - d];
+  }
+}
+
+__device__ __forceinline__ void row_score_raw(const RowLds& q, const RowRaw& R, RowBlk& S) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    S.c[u] = row_score(q, R.rec[u], R.cr[u], R.r2[u]);
+    S.z[u] = R.rec[u].z;
+    S.w[u] = R.rec[u].w;
+  }
+}
+
+// Chain of one aligned 8-column block; its forward values leave as one 16-byte
+// word per lane (hi16 of consecutive steps packed low, high).
+__device__ __forceinline__ uint32_t row_chain_fwd(const RowBlk& S, uint32_t t2,
+                                                  DE_GLOBAL uint8_t* fw) {
+  uint32_t t[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    t2 = rf_chain_z(S.z[u], t2, __umul24(S.w[u], S.c[u]) + 32768u);
+    t[u] = t2;
+  }
+  uint4 v;
+  v.x = __builtin_amdgcn_perm(t[1], t[0], 0x07060302u);
+  v.y = __builtin_amdgcn_perm(t[3], t[2], 0x07060302u);
+  v.z = __builtin_amdgcn_perm(t[5], t[4], 0x07060302u);
+  v.w = __builtin_amdgcn_perm(t[7], t[6], 0x07060302u);
+  *reinterpret_cast<DE_GLOBAL uint4*>(fw) = v;
+  return t2;
+}
+
+// Row-pass context of one wave: volume row, forward scratch, transpose tile.
+struct RowOut {
+  DE_GLOBAL uint8_t* vrow;   // volume, column 0 of this row and group (wave-uniform)
+  DE_GLOBAL uint8_t* frow;   // forward scratch of this row and group
+  uint16_t* tile;            // LDS, [8][64] u16
+  uint32_t lane;
+  int W;
+};
+
+// Final values of columns 8b..8b+7 from the tile to the volume (16 bytes per
+// lane); with CHECK, columns at or beyond W are not written.
+template <uint32_t XB, bool CHECK>
+__device__ __forceinline__ void row_flush(const RowOut& o, int b) {
+  const uint4 v = *reinterpret_cast<const uint4*>(o.tile + o.lane * 8);
+  const int xc = 8 * b + int(o.lane >> 3);
+  if (!CHECK || xc < o.W)
+    *reinterpret_cast<DE_GLOBAL uint4*>(o.vrow + size_t(xc) * XB + (o.lane & 7) * 16) = v;
+}
+
+// Coefficient words (z, w) of columns x0+1 .. x0+8, read before any tile write
+// of the block that uses them (the compiler cannot tell the tile from the
+// records, so a read issued after a tile write would wait for it).
+__device__ __forceinline__ void row_coef8(uint2 (&zw)[8], const uint4* lr, int x0) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) zw[j] = *reinterpret_cast<const uint2*>(&lr[x0 + j + 1].z);
+}
+
+// H-bwd over P interior blocks b, b-1, ..., b-P+1 from their loaded forward
+// words, coefficient of column x+1 for column x. The next block's
+// coefficients are read while the current block runs.
+template <int P, uint32_t XB>
+__device__ __forceinline__ uint32_t row_back_blocks(const uint4 (&v)[P], uint32_t t2,
+                                                    const uint4* lr, int b, const RowOut& o) {
+  uint2 zc[8], zn[8];
+  row_coef8(zc, lr, 8 * b);
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    if (k + 1 < P) row_coef8(zn, lr, 8 * (b - k - 1));
+    const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+    for (int j = 7; j >= 0; --j) {
+      const uint32_t f = (j & 1) ? (wv[j >> 1] >> 16) : (wv[j >> 1] & 0xFFFFu);
+      t2 = rf_chain_z(zc[j].x, t2, __umul24(zc[j].y, f) + 32768u);
+      o.tile[j * kWave + o.lane] = uint16_t(t2 >> 16);
+    }
+    row_flush<XB, false>(o, b - k);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) zc[j] = zn[j];
+  }
+  return t2;
+}
+
+template <int P>
+__device__ __forceinline__ void row_load_blocks(uint4 (&v)[P], const RowOut& o, int b) {
+#pragma unroll
+  for (int k = 0; k < P; ++k)
+    v[k] = *reinterpret_cast<const DE_GLOBAL uint4*>(o.frow + size_t(b - k) * 1024 + o.lane * 16);
+}
+
+// G: disparity groups of every frame in the launch; P: H-bwd prefetch depth in
+// 8-column blocks.
+template <int G, int P>
+__global__ __launch_bounds__(64) void k_row(BatchView b, int base) {
+  constexpr int Dp = G * kWave;
+  constexpr uint32_t XB = 2u * Dp;                       // bytes per volume column
   extern __shared__ __align__(16) uint8_t smem[];
-  const int item = blockIdx.x;
+  const int item = blockIdx.x + base;
   const int f = find_frame(b, kStageRow, item);
   const FrameDesc F = b.frames[f];
   const int local = local_item(b, kStageRow, item, f);
-  const int W = F.W, H = F.H, Dp = F.Dpad, G = F.G;
+  const int W = F.W, H = F.H;
   const int y = local / G, g = local - y * G;
   const uint32_t lane = threadIdx.x & 63;
+  const uint32_t vo = 2u * lane;                         // lane's byte offset in a column
   const int d = F.dmin + g * kWave + int(lane);
-  // Wave-uniform pointer to column x of this row and group; lane l owns
-  // p[l]. Every access is scalar base + lane offset, stepped by Dp.
-  DE_GLOBAL uint16_t* const x0p = gptr(F.vol) + size_t(y) * W * Dp + g * kWave;
+  RowOut o;
+  o.vrow = reinterpret_cast<DE_GLOBAL uint8_t*>(gptr(F.vol) + (size_t(y) * W * Dp + g * kWave));
+  o.lane = lane;
+  o.W = W;
   const int xlo = F.hw + F.dmin + g * kWave;             // lane 0 valid from here
   const int xend = W - F.hw;                             // left pixel valid below
   if (y < F.hh || y >= H - F.hh || xend <= xlo) {
     // every score of this row is zero -> the filtered row is zero (v' = kOff)
-    DE_GLOBAL uint16_t* p = x0p;
-    for (int x = 0; x < W; ++x, p += Dp) p[lane] = uint16_t(kOff);
+    for (int x = 0; x < W; ++x)
+      *reinterpret_cast<DE_GLOBAL uint16_t*>(o.vrow + size_t(x) * XB + vo) = uint16_t(kOff);
     return;
   }
   const int nb = (2 * F.hw + 1) * (2 * F.hh + 1) - 1;
-  const int ntbl = (nb + 1) << F.sc_shift;
-  // LDS: score table | left records (uint4) | right census (u64) | 2*R (u16)
-  uint4* lr = reinterpret_cast<uint4*>(smem + ((ntbl * 2 + 15) & ~15));
+  const int T = F.ad_T;
+  const int ntt = (T + 1) * int(kTblRow / 2);            // transposed table entries
+  // LDS: score table (transposed) | transpose tile (8 x 64 u16) | left records
+  // (uint4) | right census (u64) | kTblRow*R (u32)
+  o.tile = reinterpret_cast<uint16_t*>(smem + ((ntt * 2 + 15) & ~15));
+  uint4* lr = reinterpret_cast<uint4*>(o.tile + 8 * kWave);
   uint64_t* crs = reinterpret_cast<uint64_t*>(lr + W);
-  uint16_t* r2s = reinterpret_cast<uint16_t*>(crs + W);
+  uint32_t* r2s = reinterpret_cast<uint32_t*>(crs + W);
   {
+    // entry (a, h) = score of census distance h, intensity difference a <= T;
+    // the frame's table holds it at (h << sc_shift) + a
     uint16_t* tbl = reinterpret_cast<uint16_t*>(smem);
     const DE_CONST uint16_t* sc = cptr(F.sc);
-#pragma unroll 8
-    for (int i = lane; i < ntbl; i += kWave) tbl[i] = sc[i];
+    for (int i = lane; i < ntt; i += kWave) {
+      const int a = i / int(kTblRow / 2), h = i - a * int(kTblRow / 2);
+      tbl[i] = h <= nb ? sc[(h << F.sc_shift) + a] : uint16_t(kOff);
+    }
     const DE_CONST uint4* lrg = cptr(reinterpret_cast<const uint4*>(F.lrec)) + size_t(y) * W;
     const DE_CONST uint64_t* crg = cptr(F.cr) + size_t(y) * W;
     const DE_CONST uint8_t* Rg = cptr(F.R) + size_t(y) * W;
 #pragma unroll 8
     for (int x = lane; x < W; x += kWave) {
-      lr[x] = lrg[x];
+      uint4 rec = lrg[x];
+      // low half 2L -> kTblRow * L (<= 33660), high half (2a) kept
+      rec.z = (rec.z & 0xFFFF0000u) | ((rec.z & 0xFFFFu) * (kTblRow / 2));
+      lr[x] = rec;
       crs[x] = crg[x];
-      r2s[x] = uint16_t(2 * Rg[x]);
+      r2s[x] = kTblRow * Rg[x];
     }
   }
   __syncthreads();
-  const int sh = F.sc_shift + 1;
-  const uint32_t T2 = 2u * uint32_t(F.ad_T);
+  RowLds q;
+  q.lr = lr;
+  q.crs = crs;
+  q.r2s = r2s;
+  q.Tb = kTblRow * uint32_t(T);
+  const int Wb = (W + 7) >> 3;
+  o.frow = reinterpret_cast<DE_GLOBAL uint8_t*>(gptr(F.fwd)) + size_t(y * G + g) * Wb * 1024;
+  const uint32_t fo = 16u * lane;                        // lane's bytes within a block
+  auto fwd_at = [&](int x) {
+    return reinterpret_cast<DE_GLOBAL uint16_t*>(o.frow + size_t(x >> 3) * 1024 + fo +
+                                                 (x & 7) * 2);
+  };
   const int xa = xlo;                                    // xlo < xend here
   const int xb = max(xa, min(xlo + kWave - 1, xend));
 
-  // ---- H-fwd: x in [0, xa) every score is zero, so f' stays kOff ----
-  DE_GLOBAL uint16_t* pw = x0p;
-  for (int x = 0; x < xa; ++x, pw += Dp) pw[lane] = uint16_t(kOff);
+  // ---- H-fwd. x in [0, xa): every score is zero, so f' stays kOff (not
+  // stored: the backward pass uses kOff there) ----
   uint32_t t2 = uint32_t(kOff) << 16;
-  for (int x = xa; x < xb; ++x, pw += Dp) {              // ragged band: per-lane window
+  for (int x = xa; x < xb; ++x) {                        // ragged band: per-lane window
     const uint4 rec = lr[x];
     const bool ok = x >= F.hw + d;
-    const uint32_t c = ok ? score_at(smem, crs, r2s, rec, ok ? x - d : 0, sh, T2) : uint32_t(kOff);
-    t2 = rf_step_rec(rec, t2, c);
-    pw[lane] = hi16(t2);
+    const int j = ok ? x - d : 0;
+    const uint32_t c = ok ? row_score(q, rec, crs[j], r2s[j]) : uint32_t(kOff);
+    t2 = rf_chain_z(rec.z, t2, __umul24(rec.w, c) + 32768u);
+    *fwd_at(x) = uint16_t(t2 >> 16);
   }
-  {
-    // scores of the next 8 columns are computed while the chain runs the
-    // current 8 (they only depend on LDS and the scalar records)
-    int x = xb;
-    if (x + 8 <= xend) {
-      uint32_t c[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) c[u] = score_at(smem, crs, r2s, lr[x + u], x + u - d, sh, T2);
-      for (; x + 16 <= xend; x += 8) {
-        uint32_t cn[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-          cn[u] = score_at(smem, crs, r2s, lr[x + 8 + u], x + 8 + u - d, sh, T2);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          t2 = rf_step_rec(lr[x + u], t2, c[u]);
-          pw[lane] = hi16(t2);
-          pw += Dp;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) c[u] = cn[u];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        t2 = rf_step_rec(lr[x + u], t2, c[u]);
-        pw[lane] = hi16(t2);
-        pw += Dp;
-      }
-      x += 8;
-    }
-    for (; x < xend; ++x, pw += Dp) {
-      const uint4 rec = lr[x];
-      t2 = rf_step_rec(rec, t2, score_at(smem, crs, r2s, rec, x - d, sh, T2));
-      pw[lane] = hi16(t2);
-    }
+  const int xm0 = min((xb + 7) & ~7, xend);              // aligned blocks [xm0, xm0 + 8 nblk)
+  const int nblk = max(0, ((xend & ~7) - xm0) >> 3);
+  int x = xb;
+  for (; x < xm0; ++x) {
+    const uint4 rec = lr[x];
+    t2 = rf_chain_z(rec.z, t2,
+                    __umul24(rec.w, row_score(q, rec, crs[x - d], r2s[x - d])) + 32768u);
+    *fwd_at(x) = uint16_t(t2 >> 16);
   }
-  for (int x = xend; x < W; ++x, pw += Dp) {             // right border: zero scores
-    t2 = rf_step_rec(lr[x], t2, uint32_t(kOff));
-    pw[lane] = hi16(t2);
+  if (nblk >= 2) {
+    // block k+2 loaded, k+1 scored, k chained; two register sets of each
+    RowRaw R0, R1;
+    RowBlk S0, S1;
+    row_load_raw(q, R0, x, d);
+    row_load_raw(q, R1, x + 8, d);
+    row_score_raw(q, R0, S0);
+    int k = 0;                    // S0 holds block k scored, R1 block k+1 loaded
+    for (; k + 4 <= nblk; k += 2, x += 16) {
+      row_load_raw(q, R0, x + 16, d);
+      row_score_raw(q, R1, S1);
+      t2 = row_chain_fwd(S0, t2, o.frow + size_t(x >> 3) * 1024 + fo);
+      row_load_raw(q, R1, x + 24, d);
+      row_score_raw(q, R0, S0);
+      t2 = row_chain_fwd(S1, t2, o.frow + size_t((x + 8) >> 3) * 1024 + fo);
+    }
+    if (nblk - k == 3) {
+      row_load_raw(q, R0, x + 16, d);
+      row_score_raw(q, R1, S1);
+      t2 = row_chain_fwd(S0, t2, o.frow + size_t(x >> 3) * 1024 + fo);
+      row_score_raw(q, R0, S0);
+      t2 = row_chain_fwd(S1, t2, o.frow + size_t((x + 8) >> 3) * 1024 + fo);
+      t2 = row_chain_fwd(S0, t2, o.frow + size_t((x + 16) >> 3) * 1024 + fo);
+      x += 24;
+    } else {
+      row_score_raw(q, R1, S1);
+      t2 = row_chain_fwd(S0, t2, o.frow + size_t(x >> 3) * 1024 + fo);
+      t2 = row_chain_fwd(S1, t2, o.frow + size_t((x + 8) >> 3) * 1024 + fo);
+      x += 16;
+    }
+  } else if (nblk == 1) {
+    RowRaw R0;
+    RowBlk S0;
+    row_load_raw(q, R0, x, d);
+    row_score_raw(q, R0, S0);
+    t2 = row_chain_fwd(S0, t2, o.frow + size_t(x >> 3) * 1024 + fo);
+    x += 8;
+  }
+  for (; x < xend; ++x) {
+    const uint4 rec = lr[x];
+    t2 = rf_chain_z(rec.z, t2,
+                    __umul24(rec.w, row_score(q, rec, crs[x - d], r2s[x - d])) + 32768u);
+    *fwd_at(x) = uint16_t(t2 >> 16);
+  }
+  for (; x < W; ++x) {                                   // right border: zero scores
+    const uint4 rec = lr[x];
+    t2 = rf_chain_z(rec.z, t2, __umul24(rec.w, uint32_t(kOff)) + 32768u);
+    *fwd_at(x) = uint16_t(t2 >> 16);
   }
 
-  // ---- H-bwd: g'[W-1] = f'[W-1]; coefficient of x+1 ----
-  int x = W - 2;
-  DE_GLOBAL uint16_t* pl = x0p + ptrdiff_t(x) * Dp;      // next forward value to load
-  DE_GLOBAL uint16_t* ps = pl;                           // next value to store
-  uint32_t cur[U];
-  // software pipeline: the U forward values of the next chunk are in flight
-  // while the current chunk runs its chain
-  const int xs = xa + U - 1;                             // chunks cover [xa, x]
-  if (x >= xs) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) { cur[u] = pl[lane]; pl -= Dp; }
-  }
-  for (; x >= xs; x -= U) {
-    uint32_t nxt[U];
-    const bool more = x - U >= xs;
-    if (more) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) { nxt[u] = pl[lane]; pl -= Dp; }
+  // ---- H-bwd: g'[W-1] = f'[W-1]; coefficient of x+1 for column x ----
+  // top block bhi (holds W-1), interior blocks [blo, bhi) from the wide
+  // prefetch, then the columns below 8*blo (forward value kOff below xa)
+  auto back1 = [&](int xc) {
+    const uint32_t fv = xc >= xa ? uint32_t(*fwd_at(xc)) : uint32_t(kOff);
+    const uint2 zw = *reinterpret_cast<const uint2*>(&lr[xc + 1].z);
+    t2 = rf_chain_z(zw.x, t2, __umul24(zw.y, fv) + 32768u);
+    o.tile[(xc & 7) * kWave + lane] = uint16_t(t2 >> 16);
+    if ((xc & 7) == 0) row_flush<XB, true>(o, xc >> 3);
+  };
+  const int bhi = (W - 1) >> 3;
+  const int blo = min(bhi, (xa + 7) >> 3);
+  o.tile[((W - 1) & 7) * kWave + lane] = uint16_t(t2 >> 16);
+  if (((W - 1) & 7) == 0) row_flush<XB, true>(o, bhi);
+  for (x = W - 2; x >= 8 * bhi; --x) back1(x);
+  int bb = bhi - 1;                                      // next interior block
+  if (bb - P + 1 >= blo) {
+    // two register sets alternate between loading and filtering
+    uint4 A[P], B[P];
+    row_load_blocks<P>(A, o, bb);
+    for (; bb - 3 * P + 1 >= blo; bb -= 2 * P) {
+      row_load_blocks<P>(B, o, bb - P);
+      t2 = row_back_blocks<P, XB>(A, t2, lr, bb, o);
+      row_load_blocks<P>(A, o, bb - 2 * P);
+      t2 = row_back_blocks<P, XB>(B, t2, lr, bb - P, o);
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      t2 = rf_step_rec(lr[x - u + 1], t2, cur[u]);
-      ps[lane] = hi16(t2);
-      ps -= Dp;
+    if (bb - 2 * P + 1 >= blo) {
+      row_load_blocks<P>(B, o, bb - P);
+      t2 = row_back_blocks<P, XB>(A, t2, lr, bb, o);
+      t2 = row_back_blocks<P, XB>(B, t2, lr, bb - P, o);
+      bb -= 2 * P;
+    } else {
+      t2 = row_back_blocks<P, XB>(A, t2, lr, bb, o);
+      bb -= P;
     }
-    if (more) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) cur[u] = nxt[u];
-    }
   }
-  for (; x >= xa; --x, ps -= Dp) {
-    const uint32_t c = ps[lane];
-    t2 = rf_step_rec(lr[x + 1], t2, c);
-    ps[lane] = hi16(t2);
+  for (; bb >= blo; --bb) {
+    uint4 v[1];
+    row_load_blocks<1>(v, o, bb);
+    t2 = row_back_blocks<1, XB>(v, t2, lr, bb, o);
   }
-  for (; x >= 0; --x, ps -= Dp) {                        // forward value is kOff here
-    t2 = rf_step_rec(lr[x + 1], t2, uint32_t(kOff));
-    ps[lane] = hi16(t2);
-  }
+  for (x = 8 * blo - 1; x >= 0; --x) back1(x);
 }
 
 // ---------------------------------------------------------------------------
@@ -483,7 +711,11 @@ __global__ __launch_bounds__(256) void k_census_plane(const uint8_t* __restrict_
 // Host launchers (called from the C-ABI translation unit).
 size_t row_lds_bytes(int W, int nbits, int sc_shift) {
   const int ntbl = (nbits + 1) << sc_shift;
-  return size_t((ntbl * 2 + 15) & ~15) + size_t(W) * (16 + 8 + 2);
+  // table (the row pass stages it transposed: at most 2^sc_shift rows of 132
+  // bytes; the masked row pass keeps this layout) | row-pass transpose tile |
+  // records, right census, intensity
+  const size_t tb = std::max(size_t(ntbl) * 2, size_t(132) << sc_shift);
+  return ((tb + 15) & ~size_t(15)) + 1024 + size_t(W) * (16 + 8 + 4);
 }
 
 hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_t s) {
@@ -494,15 +726,30 @@ hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_
 }
 
 // Prefetch-depth variants (chosen per shape by the autotuner, de_tune.cpp).
-const int kRowVariants[kNumRowVariants] = {8, 16, 32, 48};
+const int kRowVariants[kNumRowVariants] = {2, 4, 8, 16};   // H-bwd blocks of 8 in flight
 const int kColVariants[kNumColVariants] = {8, 16, 32};
 
-hipError_t launch_row(const BatchView& b, int items, size_t lds, int variant, hipStream_t s) {
+template <int G>
+static void launch_row_g(const BatchView& b, int base, int items, size_t lds, int variant,
+                         hipStream_t s) {
   switch (variant) {
-    case 0: hipLaunchKernelGGL(k_row<8>, dim3(items), dim3(64), lds, s, b); break;
-    case 1: hipLaunchKernelGGL(k_row<16>, dim3(items), dim3(64), lds, s, b); break;
-    case 2: hipLaunchKernelGGL(k_row<32>, dim3(items), dim3(64), lds, s, b); break;
-    case 3: hipLaunchKernelGGL(k_row<48>, dim3(items), dim3(64), lds, s, b); break;
+    case 0: hipLaunchKernelGGL((k_row<G, 2>), dim3(items), dim3(64), lds, s, b, base); break;
+    case 1: hipLaunchKernelGGL((k_row<G, 4>), dim3(items), dim3(64), lds, s, b, base); break;
+    case 2: hipLaunchKernelGGL((k_row<G, 8>), dim3(items), dim3(64), lds, s, b, base); break;
+    default: hipLaunchKernelGGL((k_row<G, 16>), dim3(items), dim3(64), lds, s, b, base); break;
+  }
+}
+
+// Row pass over the frames whose row items are [base, base + items); every one
+// of them must have exactly G disparity groups.
+hipError_t launch_row(const BatchView& b, int base, int items, int G, size_t lds, int variant,
+                      hipStream_t s) {
+  if (variant < 0 || variant >= kNumRowVariants) return hipErrorInvalidValue;
+  switch (G) {
+    case 1: launch_row_g<1>(b, base, items, lds, variant, s); break;
+    case 2: launch_row_g<2>(b, base, items, lds, variant, s); break;
+    case 3: launch_row_g<3>(b, base, items, lds, variant, s); break;
+    case 4: launch_row_g<4>(b, base, items, lds, variant, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
