@@ -525,23 +525,47 @@ __global__ __launch_bounds__(64) void k_row(BatchView b, int base) {
 // filtered, every lane drops its final value into an LDS block (row-major,
 // groups 2w and 2w+1 packed in one word, row stride padded by one word so the
 // transposed reads hit distinct banks). After a barrier the block is read
-// transposed: each row is owned by one lane, which runs the top-2 over that
-// row's disparities on packed keys ((v'+1) << 8 | 255-k). So ties resolve to
-// the smaller k, exactly as the reference's ascending-k strictly-greater
-// insertion does. The neighbours k*-1 and k*+1 are two more LDS reads. There
-// is no cross-lane reduction. The V-bwd values never go back to memory; the
-// output is one 16-byte record per pixel.
+// transposed: each row is owned by S lanes (S = 1, 2, 2, 4 for G = 1..4, so
+// that every lane of the phase has work), each running the top-2 over its
+// share of the row's disparities on packed keys ((v'+1) << 8 | 255-k); the
+// shares merge through two lane shuffles per round. Keys order by value then
+// smaller k, exactly as the reference's ascending-k strictly-greater
+// insertion does, so the merge order does not matter. The neighbours k*-1 and
+// k*+1 are two more LDS reads. The V-bwd values never go back to memory
+// unless the volume is kept; the output is one 16-byte record per pixel.
 //
-// The volume carries kVolPadRows rows of padding above and below the frame,
-// so the prefetch runs U..2U rows ahead without bounds checks (the padding
-// is read, never used).
+// Volume accesses are buffer instructions: a wave-uniform resource based at
+// the first row of a chunk, the lane's byte offset, and the row step u * (row
+// bytes) as a scalar offset the compiler keeps in SGPRs, so a row step costs
+// one memory instruction and no address arithmetic. The volume carries
+// kVolPadRows rows of padding above and below the frame, so the prefetch runs
+// up to 2U rows past either end without bounds checks (the padding is read,
+// never used).
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t col_rsrc(DE_GLOBAL uint8_t* p) {
+  // raw buffer (stride 0); word 3 is the gfx9 data-format field of a raw
+  // dword buffer; no range check is relied on (every access is in the volume)
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(p), 0, 0x7FFFFFFF,
+                                           0x00020000);
+}
+
+// top-2 of the union of two top-2 lists of packed keys
+__device__ __forceinline__ void top2_merge(uint32_t& p0, uint32_t& p1, uint32_t q0, uint32_t q1) {
+  const uint32_t n1 = max(min(p0, q0), max(p1, q1));
+  p0 = max(p0, q0);
+  p1 = n1;
+}
 
 template <int G, int U, bool STORE>
 __global__ __launch_bounds__(64 * G) void k_col(BatchView b, int base) {
   constexpr int NWL = (G + 1) / 2;           // LDS words per lane per row
   constexpr int RS = kWave * NWL + 1;        // padded row stride (words)
   constexpr int RPW = (kWave + G - 1) / G;   // rows per wave in the transposed phase
-  static_assert(kWave % U == 0 && 2 * U <= kVolPadRows, "prefetch depth");
+  constexpr int S = G == 1 ? 1 : (G == 4 ? 4 : 2);   // lanes per row
+  constexpr int NW = kWave * NWL;            // words per row
+  constexpr int WPS = NW / S;                // words per lane
+  static_assert(RPW * S <= kWave && NW % S == 0, "transposed phase shape");
+  static_assert(kWave % (2 * U) == 0 && 2 * U <= kVolPadRows, "prefetch depth");
   __shared__ uint32_t rb[kWave * RS];
   const int item = blockIdx.x + base;
   const int f = find_frame(b, kStageCol, item);
@@ -550,44 +574,52 @@ __global__ __launch_bounds__(64 * G) void k_col(BatchView b, int base) {
   const int W = F.W, H = F.H, Dp = F.Dpad;
   const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
   const uint32_t lane = threadIdx.x & 63;
-  const ptrdiff_t ys = ptrdiff_t(W) * Dp;                // row stride (elements)
-  // wave-uniform base of this column and group; lane l reads base[l]
-  DE_GLOBAL uint16_t* const col = gptr(F.vol) + size_t(x) * Dp + g * kWave;
+  const uint32_t vo = 2u * lane;                         // lane's byte offset in a row
+  const int ysb = W * Dp * 2;                            // bytes per volume row
+  // wave-uniform address of row 0 of this column and group
+  DE_GLOBAL uint8_t* const colb =
+      reinterpret_cast<DE_GLOBAL uint8_t*>(gptr(F.vol) + (size_t(x) * Dp + g * kWave));
+  auto rs = [&](int y0) { return col_rsrc(colb + ptrdiff_t(y0) * ysb); };
   const DE_CONST uint32_t* ay = cptr(F.ayt) + size_t(x) * H;
   uint16_t* const rb16 = reinterpret_cast<uint16_t*>(rb);
   const int slot = 2 * ((g >> 1) * kWave + lane) + (g & 1);   // u16 index within a row
-  uint32_t t = uint32_t(col[lane]) << 16;
+  uint32_t t = uint32_t(__builtin_amdgcn_raw_buffer_load_b16(rs(0), vo, 0, 0)) << 16;
 
-  // ---- V-fwd over rows 1..H-1 ----
+  // ---- V-fwd over rows 1..H-1, U rows in flight ----
   {
-    DE_GLOBAL uint16_t* pf = col + ys;                   // prefetch pointer
-    DE_GLOBAL uint16_t* ps = col + ys;                   // store pointer
-    uint32_t cur[U];
+    auto ldf = [&](uint32_t (&v)[U], int y0) {           // rows y0 .. y0+U-1
+      const __amdgpu_buffer_rsrc_t rr = rs(y0);
 #pragma unroll
-    for (int u = 0; u < U; ++u) { cur[u] = pf[lane]; pf += ys; }
-    int y = 1;
-    for (; y + U <= H; y += U) {
-      uint32_t nxt[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) { nxt[u] = pf[lane]; pf += ys; }
+      for (int u = 0; u < U; ++u) v[u] = __builtin_amdgcn_raw_buffer_load_b16(rr, vo, u * ysb, 0);
+    };
+    auto fwd = [&](const uint32_t (&v)[U], int y0, int n) {
+      const __amdgpu_buffer_rsrc_t rr = rs(y0);
+      uint32_t ab[U];                                    // coefficients of the chunk, one
+      __builtin_memcpy(ab, ay + y0, sizeof(ab));         // wide scalar load
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const uint32_t a2 = ay[y + u];
-        t = rf_chain(a2, t, rf_acc(a2, cur[u]));
-        ps[lane] = hi16(t);
-        ps += ys;
+        if (u < n) {
+          const uint32_t a2 = ab[u];
+          t = rf_chain(a2, t, rf_acc(a2, v[u]));
+          __builtin_amdgcn_raw_buffer_store_b16(uint16_t(t >> 16), rr, vo, u * ysb, 0);
+        }
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+    };
+    uint32_t A[U], B[U];
+    int y = 1;
+    ldf(A, y);
+    for (; y + 2 * U <= H; y += 2 * U) {
+      ldf(B, y + U);
+      fwd(A, y, U);
+      ldf(A, y + 2 * U);
+      fwd(B, y + U, U);
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {                        // tail (< U rows)
-      if (y + u < H) {
-        const uint32_t a2 = ay[y + u];
-        t = rf_chain(a2, t, rf_acc(a2, cur[u]));
-        ps[lane] = hi16(t);
-        ps += ys;
-      }
+    if (y + U <= H) {
+      ldf(B, y + U);
+      fwd(A, y, U);
+      fwd(B, y + U, H - y - U);
+    } else {
+      fwd(A, y, H - y);
     }
   }
 
@@ -596,53 +628,82 @@ __global__ __launch_bounds__(64 * G) void k_col(BatchView b, int base) {
   const int ylo = F.hh, yhi = H - F.hh;                  // rows with candidates
   DE_GLOBAL uint4* rec = gptr(reinterpret_cast<uint4*>(F.cand)) + x;   // row y at rec[y*W]
   if (threadIdx.x == 0) rec[size_t(H - 1) * W] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
-  DE_GLOBAL uint16_t* pf = col + ptrdiff_t(H - 2) * ys;
-  DE_GLOBAL uint16_t* ps = pf;
-  uint32_t cur[U];
+  // rows ytop, ytop-1, ..., ytop-U+1 (resource based at the lowest row index)
+  auto ldb = [&](uint32_t (&v)[U], int ytop) {
+    const __amdgpu_buffer_rsrc_t rr = rs(ytop - U + 1);
 #pragma unroll
-  for (int u = 0; u < U; ++u) { cur[u] = pf[lane]; pf -= ys; }
+    for (int u = 0; u < U; ++u)
+      v[u] = __builtin_amdgcn_raw_buffer_load_b16(rr, vo, (U - 1 - u) * ysb, 0);
+  };
+  auto bwd = [&](const uint32_t (&v)[U], int ytop, int u0, int n) {
+    const __amdgpu_buffer_rsrc_t rr = rs(ytop - U + 1);
+    // coefficient of row ytop-u+1 = ab[U-1-u]: one wide scalar load per chunk
+    uint32_t ab[U];
+    __builtin_memcpy(ab, ay + (ytop - U + 2), sizeof(ab));
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (u < n) {
+        const uint32_t a2 = ab[U - 1 - u];
+        t = rf_chain(a2, t, rf_acc(a2, v[u]));
+        const uint32_t vv = t >> 16;
+        if (STORE) __builtin_amdgcn_raw_buffer_store_b16(uint16_t(vv), rr, vo, (U - 1 - u) * ysb, 0);
+        rb16[(u0 + u) * (2 * RS) + slot] = uint16_t(vv);
+      }
+    }
+  };
+  // transposed-phase ownership: lane = sp * RPW + ri owns share sp of row g*RPW + ri
+  const int ri = int(lane) % RPW, sp = int(lane) / RPW;
+  const int r = g * RPW + ri;
+  const bool kfull = kmax >= NWL * 128 - 1;              // every word of the row valid
+  const int nw = min(NW, (max(kmax, 0) >> 7) * kWave + kWave);
+  uint32_t A[U], B[U];
+  ldb(A, H - 2);
   for (int yb = H - 2; yb >= 0; yb -= kWave) {          // block: rows yb, yb-1, ..., yb-63
     const int nrow = min(kWave, yb + 1);
     int u0 = 0;
-    for (; u0 + U <= nrow; u0 += U) {
-      uint32_t nxt[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) { nxt[u] = pf[lane]; pf -= ys; }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const uint32_t a2 = ay[yb - u0 - u + 1];
-        t = rf_chain(a2, t, rf_acc(a2, cur[u]));
-        const uint32_t v = t >> 16;
-        if (STORE) ps[lane] = uint16_t(v);
-        ps -= ys;
-        rb16[(u0 + u) * (2 * RS) + slot] = uint16_t(v);
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+    for (; u0 + 2 * U <= nrow; u0 += 2 * U) {
+      ldb(B, yb - u0 - U);
+      bwd(A, yb - u0, u0, U);
+      ldb(A, yb - u0 - 2 * U);
+      bwd(B, yb - u0 - U, u0 + U, U);
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {                        // tail of the last block
-      if (u0 + u < nrow) {
-        const uint32_t a2 = ay[yb - u0 - u + 1];
-        t = rf_chain(a2, t, rf_acc(a2, cur[u]));
-        const uint32_t v = t >> 16;
-        if (STORE) ps[lane] = uint16_t(v);
-        ps -= ys;
-        rb16[(u0 + u) * (2 * RS) + slot] = uint16_t(v);
+    if (u0 < nrow) {                                     // last block: fewer than 2U rows left
+      if (u0 + U <= nrow) {
+        ldb(B, yb - u0 - U);
+        bwd(A, yb - u0, u0, U);
+        bwd(B, yb - u0 - U, u0 + U, nrow - u0 - U);
+      } else {
+        bwd(A, yb - u0, u0, nrow - u0);
       }
     }
     __syncthreads();
-    // transposed: wave g, lane r < RPW owns row yb - (g*RPW + r)
-    const int r = g * RPW + lane;
-    if (lane < RPW && r < nrow) {
-      const int yy = yb - r;
-      uint32_t p0 = 0u, p1 = 0u, nbw = 0xFFFFFFFFu;
-      if (yy >= ylo && yy < yhi && kmax >= 0) {
-        const uint32_t* row = rb + r * RS;
-        const int nw = min(NWL * kWave, (kmax >> 7) * kWave + kWave);
-        for (int j = 0; j < nw; ++j) {
-          const uint32_t wv = row[j];
+    const int yy = yb - r;
+    const bool own = sp < S && r < nrow;
+    const bool cand = own && yy >= ylo && yy < yhi && kmax >= 0;
+    uint32_t p0 = 0u, p1 = 0u;
+    if (cand) {
+      const uint32_t* row = rb + r * RS + sp * WPS;
+      if (kfull) {
+#pragma unroll 8
+        for (int jj = 0; jj < WPS; ++jj) {
+          const int j = sp * WPS + jj;
+          const uint32_t wv = row[jj];
           const int klo = ((j >> 6) << 7) + (j & 63);   // word j holds k = klo and klo + 64
+          const uint32_t a = (wv << 8 & 0xFFFF00u) + uint32_t(256 + 255 - klo);
+          const uint32_t c = (wv >> 8 & 0xFFFF00u) + uint32_t(256 + 255 - 64 - klo);
+          p1 = med3(p0, p1, a);
+          p0 = max(p0, a);
+          p1 = med3(p0, p1, c);
+          p0 = max(p0, c);
+        }
+      } else {
+        // left-border columns (kmax below the group range): bounds per key
+#pragma unroll 2
+        for (int jj = 0; jj < WPS; ++jj) {
+          const int j = sp * WPS + jj;
+          if (j >= nw) break;
+          const uint32_t wv = row[jj];
+          const int klo = ((j >> 6) << 7) + (j & 63);
           const uint32_t key_lo = (wv << 8 & 0xFFFF00u) + uint32_t(256 + 255 - klo);
           const uint32_t key_hi = (wv >> 8 & 0xFFFF00u) + uint32_t(256 + 255 - 64 - klo);
           const uint32_t a = klo <= kmax ? key_lo : 0u;
@@ -652,18 +713,29 @@ __global__ __launch_bounds__(64 * G) void k_col(BatchView b, int base) {
           p1 = med3(p0, p1, c);
           p0 = max(p0, c);
         }
-        if (p0) {
-          const int ks = key_k(p0);
-          auto val = [&](int k) -> uint32_t {
-            const uint32_t wv = row[((k >> 7) << 6) + (k & 63)];
-            return ((k >> 6) & 1) ? (wv >> 16) : (wv & 0xFFFFu);
-          };
-          const uint32_t nm = ks >= 1 ? val(ks - 1) : 0xFFFFu;
-          const uint32_t np = ks + 1 <= kmax ? val(ks + 1) : 0xFFFFu;
-          nbw = nm | (np << 16);
-        }
       }
-      rec[size_t(yy) * W] = make_uint4(p0, p1, nbw, 0u);
+    }
+    // merge the S shares of each row (every lane takes part in the shuffles)
+#pragma unroll
+    for (int m = 1; m < S; m <<= 1) {
+      const int src = ((sp ^ m) < S ? (sp ^ m) : sp) * RPW + ri;
+      const uint32_t q0 = __shfl(p0, src), q1 = __shfl(p1, src);
+      top2_merge(p0, p1, q0, q1);
+    }
+    if (own && sp == 0) {
+      uint32_t nbw = 0xFFFFFFFFu;
+      if (cand && p0) {
+        const uint32_t* row = rb + r * RS;
+        const int ks = key_k(p0);
+        auto val = [&](int k) -> uint32_t {
+          const uint32_t wv = row[((k >> 7) << 6) + (k & 63)];
+          return ((k >> 6) & 1) ? (wv >> 16) : (wv & 0xFFFFu);
+        };
+        const uint32_t nm = ks >= 1 ? val(ks - 1) : 0xFFFFu;
+        const uint32_t np = ks + 1 <= kmax ? val(ks + 1) : 0xFFFFu;
+        nbw = nm | (np << 16);
+      }
+      rec[size_t(yy) * W] = make_uint4(cand ? p0 : 0u, cand ? p1 : 0u, nbw, 0u);
     }
     __syncthreads();
   }
