@@ -23,6 +23,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "de_batch.h"
 #include "de_common.h"
@@ -189,6 +190,7 @@ __device__ __forceinline__ uint32_t bcnt_acc(uint32_t a, uint32_t acc) {
 }
 
 struct RowLds {
+  uint32_t hb;               // this wave's table address / 2 (added by the first bcnt)
   const uint4* lr;           // left records of the row (z low half = kTblRow * L)
   const uint64_t* crs;       // right census of the row
   const uint32_t* r2s;       // kTblRow * R of the row (a word each)
@@ -204,17 +206,18 @@ struct RowLds {
 // saturated lane on one bank of each row.
 constexpr uint32_t kTblRow = 132;
 
-// Score-table read; the table sits at LDS address 0 (k_row declares no static
-// LDS, so its dynamic LDS starts there) and the byte offset kTblRow*min(|L - R|,
-// T) + 2h is the address itself.
-__device__ __forceinline__ uint32_t row_tbl(uint32_t h, uint32_t adb) {
+// Score-table read. Each wave's table starts its LDS region, at address
+// 2*hb (k_row declares no static LDS, so its dynamic LDS starts at address 0);
+// hb enters as the first popcount's accumulator, so (h + hb) * 2 +
+// kTblRow*min(|L - R|, T) is the address itself: one shift-add, no base add.
+__device__ __forceinline__ uint32_t row_tbl(uint32_t hh, uint32_t adb) {
   return *reinterpret_cast<const __attribute__((address_space(3))) uint16_t*>(
-      uintptr_t((h << 1) + adb));
+      uintptr_t((hh << 1) + adb));
 }
 
 __device__ __forceinline__ uint32_t row_score(const RowLds& q, const uint4& rec, uint64_t cr,
                                               uint32_t rb) {
-  const uint32_t h = bcnt_acc(rec.y ^ uint32_t(cr >> 32), __popc(rec.x ^ uint32_t(cr)));
+  const uint32_t h = bcnt_acc(rec.y ^ uint32_t(cr >> 32), __popc(rec.x ^ uint32_t(cr)) + q.hb);
   const uint32_t adb = min(sad_u32(rec.z & 0xFFFFu, rb, 0u), q.Tb);
   return row_tbl(h, adb);
 }
@@ -273,19 +276,27 @@ __device__ __forceinline__ uint32_t row_chain_fwd(const RowBlk& S, uint32_t t2,
 struct RowOut {
   DE_GLOBAL uint8_t* vrow;   // volume, column 0 of this row and group (wave-uniform)
   DE_GLOBAL uint8_t* frow;   // forward scratch of this row and group
-  uint16_t* tile;            // LDS, [8][64] u16
+  uint16_t* tile;            // LDS, two tiles of [8][64] u16
   uint32_t lane;
+  uint32_t so;               // lane's store offset within 8 columns: (lane/8)*XB + (lane%8)*16
   int W;
 };
 
-// Final values of columns 8b..8b+7 from the tile to the volume (16 bytes per
-// lane); with CHECK, columns at or beyond W are not written.
+// Final values of columns 8b..8b+7 (read from a tile: lane l holds
+// disparities 8(l%8)..8(l%8)+7 of column 8b + l/8) to the volume, 16 bytes per
+// lane.
+template <uint32_t XB>
+__device__ __forceinline__ void row_store8(const RowOut& o, int b, const uint4& v) {
+  *reinterpret_cast<DE_GLOBAL uint4*>(o.vrow + size_t(8 * b) * XB + o.so) = v;
+}
+
+// The same from tile 0, right away; with CHECK, columns at or beyond W are not
+// written.
 template <uint32_t XB, bool CHECK>
 __device__ __forceinline__ void row_flush(const RowOut& o, int b) {
   const uint4 v = *reinterpret_cast<const uint4*>(o.tile + o.lane * 8);
   const int xc = 8 * b + int(o.lane >> 3);
-  if (!CHECK || xc < o.W)
-    *reinterpret_cast<DE_GLOBAL uint4*>(o.vrow + size_t(xc) * XB + (o.lane & 7) * 16) = v;
+  if (!CHECK || xc < o.W) row_store8<XB>(o, b, v);
 }
 
 // Coefficient words (z, w) of columns x0+1 .. x0+8, read before any tile write
@@ -298,26 +309,32 @@ __device__ __forceinline__ void row_coef8(uint2 (&zw)[8], const uint4* lr, int x
 
 // H-bwd over P interior blocks b, b-1, ..., b-P+1 from their loaded forward
 // words, coefficient of column x+1 for column x. The next block's
-// coefficients are read while the current block runs.
+// coefficients are read before the current block's tile writes; the tiles
+// alternate, and a block's 16-byte stores leave after the next block's chain,
+// so neither LDS read is waited on right where it is issued.
 template <int P, uint32_t XB>
 __device__ __forceinline__ uint32_t row_back_blocks(const uint4 (&v)[P], uint32_t t2,
                                                     const uint4* lr, int b, const RowOut& o) {
   uint2 zc[8], zn[8];
+  uint4 pend = make_uint4(0u, 0u, 0u, 0u);
   row_coef8(zc, lr, 8 * b);
 #pragma unroll
   for (int k = 0; k < P; ++k) {
     if (k + 1 < P) row_coef8(zn, lr, 8 * (b - k - 1));
+    uint16_t* const tile = o.tile + (k & 1) * 8 * kWave;
     const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
 #pragma unroll
     for (int j = 7; j >= 0; --j) {
       const uint32_t f = (j & 1) ? (wv[j >> 1] >> 16) : (wv[j >> 1] & 0xFFFFu);
       t2 = rf_chain_z(zc[j].x, t2, __umul24(zc[j].y, f) + 32768u);
-      o.tile[j * kWave + o.lane] = uint16_t(t2 >> 16);
+      tile[j * kWave + o.lane] = uint16_t(t2 >> 16);
     }
-    row_flush<XB, false>(o, b - k);
+    if (k > 0) row_store8<XB>(o, b - k + 1, pend);
+    pend = *reinterpret_cast<const uint4*>(tile + o.lane * 8);
 #pragma unroll
     for (int j = 0; j < 8; ++j) zc[j] = zn[j];
   }
+  row_store8<XB>(o, b - P + 1, pend);
   return t2;
 }
 
@@ -329,13 +346,20 @@ __device__ __forceinline__ void row_load_blocks(uint4 (&v)[P], const RowOut& o, 
 }
 
 // G: disparity groups of every frame in the launch; P: H-bwd prefetch depth in
-// 8-column blocks.
+// 8-column blocks. A workgroup is 1..4 independent waves (one item each, own
+// LDS region of pw bytes): a lone wave per workgroup lets the dispatcher
+// stack several on one SIMD of a CU, while the waves of one workgroup go to
+// distinct SIMDs.
 template <int G, int P>
-__global__ __launch_bounds__(64) void k_row(BatchView b, int base) {
+__global__ __launch_bounds__(256) void k_row(BatchView b, int base, int items, int pw) {
   constexpr int Dp = G * kWave;
   constexpr uint32_t XB = 2u * Dp;                       // bytes per volume column
-  extern __shared__ __align__(16) uint8_t smem[];
-  const int item = blockIdx.x + base;
+  extern __shared__ __align__(16) uint8_t smem_all[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nwg = blockDim.x >> 6;
+  if (blockIdx.x * nwg + wv >= items) return;            // whole wave: no barriers follow
+  uint8_t* const smem = smem_all + wv * pw;
+  const int item = base + blockIdx.x * nwg + wv;
   const int f = find_frame(b, kStageRow, item);
   const FrameDesc F = b.frames[f];
   const int local = local_item(b, kStageRow, item, f);
@@ -347,6 +371,7 @@ __global__ __launch_bounds__(64) void k_row(BatchView b, int base) {
   RowOut o;
   o.vrow = reinterpret_cast<DE_GLOBAL uint8_t*>(gptr(F.vol) + (size_t(y) * W * Dp + g * kWave));
   o.lane = lane;
+  o.so = (lane >> 3) * XB + (lane & 7) * 16;
   o.W = W;
   const int xlo = F.hw + F.dmin + g * kWave;             // lane 0 valid from here
   const int xend = W - F.hw;                             // left pixel valid below
@@ -359,10 +384,10 @@ __global__ __launch_bounds__(64) void k_row(BatchView b, int base) {
   const int nb = (2 * F.hw + 1) * (2 * F.hh + 1) - 1;
   const int T = F.ad_T;
   const int ntt = (T + 1) * int(kTblRow / 2);            // transposed table entries
-  // LDS: score table (transposed) | transpose tile (8 x 64 u16) | left records
-  // (uint4) | right census (u64) | kTblRow*R (u32)
+  // LDS: score table (transposed) | two transpose tiles (8 x 64 u16) | left
+  // records (uint4) | right census (u64) | kTblRow*R (u32)
   o.tile = reinterpret_cast<uint16_t*>(smem + ((ntt * 2 + 15) & ~15));
-  uint4* lr = reinterpret_cast<uint4*>(o.tile + 8 * kWave);
+  uint4* lr = reinterpret_cast<uint4*>(o.tile + 2 * 8 * kWave);
   uint64_t* crs = reinterpret_cast<uint64_t*>(lr + W);
   uint32_t* r2s = reinterpret_cast<uint32_t*>(crs + W);
   {
@@ -387,8 +412,11 @@ __global__ __launch_bounds__(64) void k_row(BatchView b, int base) {
       r2s[x] = kTblRow * Rg[x];
     }
   }
-  __syncthreads();
+  // the wave reads back only what it wrote; LDS keeps one wave's accesses in
+  // order, so a compiler barrier suffices
+  __builtin_amdgcn_wave_barrier();
   RowLds q;
+  q.hb = uint32_t(wv * pw) >> 1;
   q.lr = lr;
   q.crs = crs;
   q.r2s = r2s;
@@ -784,10 +812,10 @@ __global__ __launch_bounds__(256) void k_census_plane(const uint8_t* __restrict_
 size_t row_lds_bytes(int W, int nbits, int sc_shift) {
   const int ntbl = (nbits + 1) << sc_shift;
   // table (the row pass stages it transposed: at most 2^sc_shift rows of 132
-  // bytes; the masked row pass keeps this layout) | row-pass transpose tile |
-  // records, right census, intensity
+  // bytes; the masked row pass keeps this layout) | two row-pass transpose
+  // tiles | records, right census, intensity
   const size_t tb = std::max(size_t(ntbl) * 2, size_t(132) << sc_shift);
-  return ((tb + 15) & ~size_t(15)) + 1024 + size_t(W) * (16 + 8 + 4);
+  return ((tb + 15) & ~size_t(15)) + 2048 + size_t(W) * (16 + 8 + 4);
 }
 
 hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_t s) {
@@ -801,30 +829,66 @@ hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_
 const int kRowVariants[kNumRowVariants] = {2, 4, 8, 16};   // H-bwd blocks of 8 in flight
 const int kColVariants[kNumColVariants] = {8, 16, 32};
 
+// Device LDS per workgroup and CU count (queried once).
+static void device_shape(int* lds, int* cus) {
+  static int l = 0, c = 0;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&l, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      l = 64 * 1024;
+      c = 1;
+    }
+  });
+  *lds = l;
+  *cus = c;
+}
+
+template <int G, int P>
+static hipError_t launch_row_gp(const BatchView& b, int base, int items, size_t pw,
+                                hipStream_t s) {
+  int lds = 0, cus = 1;
+  device_shape(&lds, &cus);
+  static const hipError_t attr = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(&k_row<G, P>), hipFuncAttributeMaxDynamicSharedMemorySize,
+      lds);
+  if (attr != hipSuccess) return attr;
+  pw = (pw + 15) & ~size_t(15);
+  // waves per workgroup: enough to give every CU's SIMDs one wave each, as
+  // many as the LDS allows, at most 4
+  const int fit = int(std::min<size_t>(4, size_t(lds) / pw));
+  const int nw = std::max(1, std::min(fit, (items + cus - 1) / cus));
+  hipLaunchKernelGGL((k_row<G, P>), dim3((items + nw - 1) / nw), dim3(64 * nw), nw * pw, s, b,
+                     base, items, int(pw));
+  return hipGetLastError();
+}
+
 template <int G>
-static void launch_row_g(const BatchView& b, int base, int items, size_t lds, int variant,
-                         hipStream_t s) {
+static hipError_t launch_row_g(const BatchView& b, int base, int items, size_t lds, int variant,
+                               hipStream_t s) {
   switch (variant) {
-    case 0: hipLaunchKernelGGL((k_row<G, 2>), dim3(items), dim3(64), lds, s, b, base); break;
-    case 1: hipLaunchKernelGGL((k_row<G, 4>), dim3(items), dim3(64), lds, s, b, base); break;
-    case 2: hipLaunchKernelGGL((k_row<G, 8>), dim3(items), dim3(64), lds, s, b, base); break;
-    default: hipLaunchKernelGGL((k_row<G, 16>), dim3(items), dim3(64), lds, s, b, base); break;
+    case 0: return launch_row_gp<G, 2>(b, base, items, lds, s);
+    case 1: return launch_row_gp<G, 4>(b, base, items, lds, s);
+    case 2: return launch_row_gp<G, 8>(b, base, items, lds, s);
+    default: return launch_row_gp<G, 16>(b, base, items, lds, s);
   }
 }
 
 // Row pass over the frames whose row items are [base, base + items); every one
-// of them must have exactly G disparity groups.
+// of them must have exactly G disparity groups. lds: LDS bytes one wave needs.
 hipError_t launch_row(const BatchView& b, int base, int items, int G, size_t lds, int variant,
                       hipStream_t s) {
   if (variant < 0 || variant >= kNumRowVariants) return hipErrorInvalidValue;
+  if (items <= 0) return hipSuccess;
   switch (G) {
-    case 1: launch_row_g<1>(b, base, items, lds, variant, s); break;
-    case 2: launch_row_g<2>(b, base, items, lds, variant, s); break;
-    case 3: launch_row_g<3>(b, base, items, lds, variant, s); break;
-    case 4: launch_row_g<4>(b, base, items, lds, variant, s); break;
+    case 1: return launch_row_g<1>(b, base, items, lds, variant, s);
+    case 2: return launch_row_g<2>(b, base, items, lds, variant, s);
+    case 3: return launch_row_g<3>(b, base, items, lds, variant, s);
+    case 4: return launch_row_g<4>(b, base, items, lds, variant, s);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 template <int G, bool STORE>
