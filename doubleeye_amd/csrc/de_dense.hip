@@ -856,10 +856,12 @@ static hipError_t launch_row_gp(const BatchView& b, int base, int items, size_t 
       lds);
   if (attr != hipSuccess) return attr;
   pw = (pw + 15) & ~size_t(15);
-  // waves per workgroup: enough to give every CU's SIMDs one wave each, as
-  // many as the LDS allows, at most 4
+  // waves per workgroup: when every wave fits in one round (at most one per
+  // SIMD), as many per workgroup as spread them one per SIMD of a CU (and the
+  // LDS allows); with more waves than SIMDs, lone-wave workgroups spread
+  // more evenly and their waves overlap on a SIMD
   const int fit = int(std::min<size_t>(4, size_t(lds) / pw));
-  const int nw = std::max(1, std::min(fit, (items + cus - 1) / cus));
+  const int nw = items <= 4 * cus ? std::max(1, std::min(fit, (items + cus - 1) / cus)) : 1;
   hipLaunchKernelGGL((k_row<G, P>), dim3((items + nw - 1) / nw), dim3(64 * nw), nw * pw, s, b,
                      base, items, int(pw));
   return hipGetLastError();
