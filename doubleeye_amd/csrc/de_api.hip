@@ -248,6 +248,7 @@ int Ctx::prepare(const FrameDesc* fds_in, int n, Prepared* P) const {
     const int nbits = (2 * fds[i].hw + 1) * (2 * fds[i].hh + 1) - 1;
     P->lds_row = std::max(P->lds_row, row_lds_bytes(fds[i].W, nbits, fds[i].sc_shift));
     P->lds_solve = std::max(P->lds_solve, solve_lds_bytes(fds[i].W));
+    P->max_w = std::max(P->max_w, fds[i].W);
     if (size_t(fds[i].W) * fds[i].H * fds[i].D > size_t(fds[big].W) * fds[big].H * fds[big].D)
       big = i;
   }
@@ -302,7 +303,7 @@ int Ctx::launch_stages(const Prepared& P, const BatchView& bv, hipStream_t s, in
       (e = launch_lrc_exact(bv, items(kStageSolve), s)) != hipSuccess)
     return hip_fail(e, "k_lrc_exact");
   if (ev) DE_HIP(hipEventRecord(ev->e[3], s));
-  if ((stage_mask & 8) && (e = launch_solve(bv, items(kStageSolve), P.lds_solve, s)) != hipSuccess)
+  if ((stage_mask & 8) && (e = launch_solve(bv, items(kStageSolve), P.lds_solve, P.max_w, s)) != hipSuccess)
     return hip_fail(e, "k_solve");
   if ((stage_mask & 8) && any_lrc &&
       (e = launch_lrc_plane(bv, items(kStageSolve), s)) != hipSuccess)

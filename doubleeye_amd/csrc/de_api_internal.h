@@ -101,6 +101,7 @@ struct Ctx {
     std::vector<FrameDesc> frames;
     std::vector<int> start;
     size_t lds_row = 0, lds_solve = 0;
+    int max_w = 0;                // widest row (the solve launch's pixels per thread)
     int rowv = 0, colv = 0;
     size_t desc_bytes() const { return frames.size() * sizeof(FrameDesc) + start.size() * sizeof(int); }
     void pack(void* dst) const;
@@ -121,7 +122,7 @@ hipError_t launch_row(const BatchView& b, int base, int items, int G, size_t lds
                       hipStream_t s);
 hipError_t launch_col(const BatchView& b, int base, int items, int G, bool store, int variant,
                       hipStream_t s);
-hipError_t launch_solve(const BatchView& b, int items, size_t lds, hipStream_t s);
+hipError_t launch_solve(const BatchView& b, int items, size_t lds, int maxW, hipStream_t s);
 hipError_t launch_lrc_exact(const BatchView& b, int items, hipStream_t s);
 hipError_t launch_lrc_plane(const BatchView& b, int items, hipStream_t s);
 hipError_t launch_census_plane(const uint8_t* img, int W, int H, int hw, int hh,

@@ -151,7 +151,7 @@ static int prior_dev(Ctx& x, const uint8_t* dL, const uint8_t* dR, int W, int H,
   if ((e = launch_masked(P.frames[0], bv.frames, d_prior, band, m, P.lds_row, fd.lrc != 0, s)) !=
       hipSuccess)
     return hip_fail(e, "masked sweep");
-  if ((e = launch_solve(bv, items(kStageSolve), P.lds_solve, s)) != hipSuccess)
+  if ((e = launch_solve(bv, items(kStageSolve), P.lds_solve, P.max_w, s)) != hipSuccess)
     return hip_fail(e, "k_solve");
   if (fd.lrc && (e = launch_lrc_plane(bv, items(kStageSolve), s)) != hipSuccess)
     return hip_fail(e, "k_lrc_plane");

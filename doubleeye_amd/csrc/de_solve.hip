@@ -31,7 +31,7 @@ namespace de {
 
 namespace {
 
-constexpr int kSolveThreads = 1024;
+constexpr int kSolveThreads = 512;   // 8 waves: two and more rows resident per CU
 
 __device__ __forceinline__ float fmax_ref(float a, float b) {   // std::max(a, b)
   return (a < b) ? b : a;
@@ -51,6 +51,8 @@ struct Top2 {
   }
   __device__ float excl(int i) const { return i == i1 ? b2 : b1; }
 };
+
+constexpr int kBucketRegs = 4;                      // bucket items kept in registers
 
 // LDS layout for one row of width W (bytes): see solve_lds_bytes().
 struct SolveSmem {
@@ -108,7 +110,11 @@ __device__ float confidence_dev(float s1, float s2, float lambda, float lrc, flo
 
 }  // namespace
 
+// KP: pixels (and buckets) per thread, at least ceil(W / kSolveThreads) for
+// every frame of the launch.
+template <int KP>
 __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
+  constexpr int kPerThread = KP;
   extern __shared__ __align__(16) uint8_t smem_raw[];
   const int item = blockIdx.x;
   const int f = find_frame(b, kStageSolve, item);
@@ -148,23 +154,35 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
   }
   __syncthreads();
 
-  // --- exclusive scan of the counts (chunk per thread, then a block scan) ---
+  // --- exclusive scan of the counts: chunk per thread, inclusive scan within
+  // each wave through lane shuffles, then over the wave totals ---
   {
     const int per = (W + nt - 1) / nt;
     const int lo = min(W, tid * per), hi = min(W, lo + per);
+    const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
     int sum = 0;
     for (int i = lo; i < hi; ++i) sum += S.bstart[i];
-    S.scan[tid] = sum;
-    __syncthreads();
-    for (int off = 1; off < nt; off <<= 1) {
-      const int v = tid >= off ? S.scan[tid - off] : 0;
-      __syncthreads();
-      S.scan[tid] += v;
-      __syncthreads();
+    int v = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int u = __shfl_up(v, off);
+      if (lane >= off) v += u;
     }
-    int run = S.scan[tid] - sum;
+    if (lane == 63) S.scan[wave] = v;
+    __syncthreads();
+    if (wave == 0) {
+      int w = lane < nwave ? S.scan[lane] : 0;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(w, off);
+        if (lane >= off) w += u;
+      }
+      if (lane < nwave) S.scan[lane] = w;
+    }
+    __syncthreads();
+    int run = (wave > 0 ? S.scan[wave - 1] : 0) + v - sum;
     for (int i = lo; i < hi; ++i) { const int c = S.bstart[i]; S.bstart[i] = run; run += c; }
-    if (tid == nt - 1) S.bstart[W] = S.scan[nt - 1];
+    if (tid == nt - 1) S.bstart[W] = S.scan[nwave - 1];
   }
   __syncthreads();
   for (int x = tid; x < W; x += nt) S.cursor[x] = S.bstart[x];
@@ -192,30 +210,118 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
   }
 
   // --- message passing ---
-  for (int it = 0; it < F.iters; ++it) {
-    for (int x = tid; x < W; x += nt) {
+  // Thread tid owns pixels x = tid + k*nt and buckets xr = tid + k*nt. A
+  // pixel's candidates, beta and rho live in its owner's registers across
+  // the iterations; rho goes to LDS for the bucket owners, which publish each
+  // bucket's top-2 of rho (best, runner-up, item index of the first best);
+  // the pixel owner then updates its beta from that. Item indices are unique
+  // in a bucket, so excluding by index equals excluding by position, and the
+  // float expressions are the reference's, operation for operation.
+  float* const bt1 = S.beta;                      // [W] bucket best rho
+  float* const bt2 = S.beta + W;                  // [W] bucket runner-up
+  int* const bti = S.cursor;                      // [W] item index of the best (-1: empty)
+  float c_s[kPerThread][2], c_b[kPerThread][2], c_r[kPerThread][2];
+  int c_xr[kPerThread][2], c_n[kPerThread];
+  int b_lo[kPerThread], b_cnt[kPerThread], b_it[kPerThread][kBucketRegs];
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const int x = tid + k * nt;
+    c_n[k] = 0;
+    b_cnt[k] = 0;
+    b_lo[k] = 0;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      c_s[k][j] = kNone; c_b[k][j] = 0.f; c_r[k][j] = 0.f; c_xr[k][j] = -1;
+    }
+#pragma unroll
+    for (int i = 0; i < kBucketRegs; ++i) b_it[k][i] = 0;
+    if (x < W) {
       const int n = S.cn[x];
-      if (!n) continue;
-      Top2 t;
-      for (int j = 0; j < n; ++j) t.push(S.beta[2 * x + j], j);
-      for (int j = 0; j < n; ++j) {
-        const float tgt = S.cs[2 * x + j] - fmax_ref(lambda, t.excl(j));
-        S.rho[2 * x + j] = keep * tgt + damping * S.rho[2 * x + j];
+      c_n[k] = n;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        if (j < n) {
+          c_s[k][j] = S.cs[2 * x + j];
+          const int xr = x - (dmin + S.cd[2 * x + j]);
+          c_xr[k][j] = (xr >= 0 && xr < W) ? xr : -1;
+        }
       }
+      const int lo = S.bstart[x], cnt = S.bstart[x + 1] - lo;
+      b_lo[k] = lo;
+      b_cnt[k] = cnt;
+#pragma unroll
+      for (int i = 0; i < kBucketRegs; ++i) if (i < cnt) b_it[k][i] = S.items[lo + i];
     }
-    __syncthreads();
-    for (int xr = tid; xr < W; xr += nt) {
-      const int lo = S.bstart[xr], hi = S.bstart[xr + 1];
-      if (hi <= lo) continue;
-      Top2 t;
-      for (int p = lo; p < hi; ++p) t.push(S.rho[S.items[p]], p - lo);
-      for (int p = lo; p < hi; ++p) {
-        const int idx = S.items[p];
-        S.beta[idx] = keep * (S.cs[idx] - fmax_ref(gamma, t.excl(p - lo))) + damping * S.beta[idx];
-      }
-    }
-    __syncthreads();
   }
+  for (int it = 0; it < F.iters; ++it) {
+    // rho of every pixel from its own betas
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+      const int n = c_n[k];
+      if (!n) continue;
+      const int x = tid + k * nt;
+      Top2 t;
+      t.push(c_b[k][0], 0);
+      if (n > 1) t.push(c_b[k][1], 1);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        if (j < n) {
+          const float tgt = c_s[k][j] - fmax_ref(lambda, t.excl(j));
+          c_r[k][j] = keep * tgt + damping * c_r[k][j];
+          S.rho[2 * x + j] = c_r[k][j];
+        }
+      }
+    }
+    __syncthreads();
+    // each bucket's top-2 of rho
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+      const int xr = tid + k * nt;
+      if (xr >= W) continue;
+      const int cnt = b_cnt[k];
+      Top2 t;
+#pragma unroll
+      for (int i = 0; i < kBucketRegs; ++i)
+        if (i < cnt) t.push(S.rho[b_it[k][i]], b_it[k][i]);
+      for (int i = kBucketRegs; i < cnt; ++i) {
+        const int idx = S.items[b_lo[k] + i];
+        t.push(S.rho[idx], idx);
+      }
+      bt1[xr] = t.b1;
+      bt2[xr] = t.b2;
+      bti[xr] = t.i1;
+    }
+    __syncthreads();
+    // beta of every bucketed candidate from its bucket's top-2
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+      const int n = c_n[k];
+      if (!n) continue;
+      const int x = tid + k * nt;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int xr = c_xr[k][j];
+        if (j < n && xr >= 0) {
+          const float ex = bti[xr] == 2 * x + j ? bt2[xr] : bt1[xr];
+          c_b[k][j] = keep * (c_s[k][j] - fmax_ref(gamma, ex)) + damping * c_b[k][j];
+        }
+      }
+    }
+  }
+  // final betas and rhos back to LDS for the belief pass (the bucket top-2
+  // arrays share the beta storage: every thread is past its last read)
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const int x = tid + k * nt;
+    if (x >= W) continue;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      S.beta[2 * x + j] = c_b[k][j];
+      S.rho[2 * x + j] = c_r[k][j];
+    }
+  }
+  __syncthreads();
 
   // --- belief, gates, claim ---
   for (int x = tid; x < W; x += nt) {
@@ -323,8 +429,14 @@ size_t solve_lds_bytes(int W) {
          r(size_t(W)) * 2;
 }
 
-hipError_t launch_solve(const BatchView& b, int items, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL(k_solve, dim3(items), dim3(kSolveThreads), lds, s, b);
+// maxW: widest row of the launch (at most kMaxW).
+hipError_t launch_solve(const BatchView& b, int items, size_t lds, int maxW, hipStream_t s) {
+  const int kp = (maxW + kSolveThreads - 1) / kSolveThreads;
+  if (kp <= 1) hipLaunchKernelGGL(k_solve<1>, dim3(items), dim3(kSolveThreads), lds, s, b);
+  else if (kp == 2) hipLaunchKernelGGL(k_solve<2>, dim3(items), dim3(kSolveThreads), lds, s, b);
+  else if (kp <= 4) hipLaunchKernelGGL(k_solve<4>, dim3(items), dim3(kSolveThreads), lds, s, b);
+  else if (kp <= 8) hipLaunchKernelGGL(k_solve<8>, dim3(items), dim3(kSolveThreads), lds, s, b);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
