@@ -191,7 +191,8 @@ size_t scratch_bytes(int W, int H, int D) {
   const int Dpad = ((D + kWave - 1) / kWave) * kWave;
   const size_t vol_rows = size_t(H) + 2 * kVolPadRows;
   const size_t fwd = size_t(H) * (Dpad / kWave) * ((W + 7) / 8) * 1024;
-  return WH * (16 + 8 + 4 + 16 + 2) + vol_rows * W * size_t(Dpad) * 2 + fwd + 7 * 256;
+  return WH * (16 + 8 + 4 + 16 + 2) + 2 * kAytPad * 4 + vol_rows * W * size_t(Dpad) * 2 + fwd +
+         7 * 256;
 }
 
 // Carve one frame's scratch out of a device block.
@@ -203,7 +204,9 @@ Scratch carve_scratch(void* base, int W, int H, int D) {
   Scratch s;
   s.lrec = reinterpret_cast<uint32_t*>(take(WH * 16));
   s.cr = reinterpret_cast<uint64_t*>(take(WH * 8));
-  s.ayt = reinterpret_cast<uint32_t*>(take(WH * 4));
+  // ayt carries kAytPad words of padding on each side: the column passes load
+  // a chunk's coefficients with one wide load that may run past a column's ends
+  s.ayt = reinterpret_cast<uint32_t*>(take(WH * 4 + 2 * kAytPad * 4)) + kAytPad;
   s.cand = reinterpret_cast<uint32_t*>(take(WH * 16));
   s.rdisp = reinterpret_cast<int16_t*>(take(WH * 2));
   // the volume keeps kVolPadRows rows of padding above and below the frame

@@ -18,6 +18,7 @@ extern const int kColVariants[kNumColVariants];
 
 // Rows of padding above and below a frame's volume (prefetch runs ahead into it).
 constexpr int kVolPadRows = 64;
+constexpr int kAytPad = 64;       // words of padding around the transposed coefficients
 
 constexpr int kPrepTileW = 64;   // census/coefficient tile: 64 x 8 pixels
 constexpr int kPrepTileH = 8;

@@ -134,20 +134,6 @@ __global__ __launch_bounds__(256) void k_prep(BatchView b) {
 // Lane = disparity. The row's left records (wave-uniform per x, broadcast
 // reads) and its right census/intensity (read at x - d per lane) are staged
 // in LDS once, with coalesced loads.
-__device__ __forceinline__ uint32_t score_at(const uint8_t* smem, const uint64_t* crs,
-                                             const uint16_t* r2s, uint4 lrec, int j,
-                                             int sh, uint32_t T2) {
-  const uint64_t cr = crs[j];
-  const uint32_t h = __popc(lrec.x ^ uint32_t(cr)) + __popc(lrec.y ^ uint32_t(cr >> 32));
-  const uint32_t ad2 = min(sad_u32(lrec.z & 0xFFFFu, uint32_t(r2s[j]), 0u), T2);
-  return *reinterpret_cast<const uint16_t*>(smem + ((h << sh) + ad2));
-}
-
-__device__ __forceinline__ uint32_t rf_step_rec(const uint4& rec, uint32_t t2, uint32_t c) {
-  // rec.z >> 16 = 2a, rec.w = 65536 - 2a (wave-uniform, read from LDS)
-  const uint32_t a2 = rec.z >> 16;
-  return rf_chain_v(a2, t2, __umul24(rec.w, c) + 32768u);
-}
 
 // The launch has about one wave per SIMD (one wave per row and group), so the
 // loops are written to hide latency inside the wave and to a budget per
@@ -1096,24 +1082,31 @@ __global__ __launch_bounds__(256) void k_mask_rect(const FrameDesc* __restrict__
   rect[k] = skip ? make_int4(0, 0, 0, 0) : make_int4(y0, y1, x0, x1);
 }
 
+// (the results are wave-uniform; readfirstlane tells the compiler, so loads
+// addressed by them are scalar)
 __device__ __forceinline__ int wave_min(int v) {
   for (int o = 1; o < kWave; o <<= 1) v = min(v, __shfl_xor(v, o));
-  return v;
+  return __builtin_amdgcn_readfirstlane(v);
 }
 __device__ __forceinline__ int wave_max(int v) {
   for (int o = 1; o < kWave; o <<= 1) v = max(v, __shfl_xor(v, o));
-  return v;
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
 // Score + H-fwd + H-bwd of one row and 64 planes, each plane on its own
 // rectangle's x range.
+template <int G>
 __global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
                                              const int4* __restrict__ rect) {
+  constexpr int Dp = G * kWave;
+  constexpr uint32_t XB = 2u * Dp;                       // bytes per volume column
+  constexpr int kU = 16;   // scores (forward) and loads (backward) in flight
   extern __shared__ __align__(16) uint8_t smem[];
   const FrameDesc F = *fp;
-  const int W = F.W, Dp = F.Dpad, G = F.G;
+  const int W = F.W;
   const int y = blockIdx.x / G, g = blockIdx.x - y * G;
   const int lane = int(threadIdx.x);
+  const uint32_t vo = 2u * uint32_t(lane);
   const int k = g * kWave + lane;
   const int d = F.dmin + k;
   const int4 r = k < F.D ? rect[k] : make_int4(0, 0, 0, 0);
@@ -1121,28 +1114,44 @@ __global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
   const int xs = act ? r.z : W, xe = act ? r.w : 0;
   const int xmin = wave_min(xs), xmax = wave_max(xe);
   if (xmin >= xmax) return;   // uniform over the wave
+  // LDS as the full row pass stages it: transposed score table at address 0,
+  // records with z's low half = kTblRow * L, right census, kTblRow * R
   const int nb = (2 * F.hw + 1) * (2 * F.hh + 1) - 1;
-  const int ntbl = (nb + 1) << F.sc_shift;
-  uint4* lr = reinterpret_cast<uint4*>(smem + ((ntbl * 2 + 15) & ~15));
+  const int T = F.ad_T;
+  const int ntt = (T + 1) * int(kTblRow / 2);
+  uint4* lr = reinterpret_cast<uint4*>(smem + ((ntt * 2 + 15) & ~15));
   uint64_t* crs = reinterpret_cast<uint64_t*>(lr + W);
-  uint16_t* r2s = reinterpret_cast<uint16_t*>(crs + W);
+  uint32_t* r2s = reinterpret_cast<uint32_t*>(crs + W);
   {
     uint16_t* tbl = reinterpret_cast<uint16_t*>(smem);
-    for (int i = lane; i < ntbl; i += kWave) tbl[i] = F.sc[i];
+    for (int i = lane; i < ntt; i += kWave) {
+      const int a = i / int(kTblRow / 2), h = i - a * int(kTblRow / 2);
+      tbl[i] = h <= nb ? F.sc[(h << F.sc_shift) + a] : uint16_t(kOff);
+    }
     const uint4* lrg = reinterpret_cast<const uint4*>(F.lrec) + size_t(y) * W;
     const uint64_t* crg = F.cr + size_t(y) * W;
     const uint8_t* Rg = F.R + size_t(y) * W;
     for (int x = lane; x < W; x += kWave) {
-      lr[x] = lrg[x];
+      uint4 rec = lrg[x];
+      rec.z = (rec.z & 0xFFFF0000u) | ((rec.z & 0xFFFFu) * (kTblRow / 2));
+      lr[x] = rec;
       crs[x] = crg[x];
-      r2s[x] = uint16_t(2 * Rg[x]);
+      r2s[x] = kTblRow * Rg[x];
     }
   }
   __syncthreads();
-  const int sh = F.sc_shift + 1;
-  const uint32_t T2 = 2u * uint32_t(F.ad_T);
-  uint16_t* const x0p = F.vol + size_t(y) * W * Dp + g * kWave;
-  constexpr int kU = 16;   // scores (forward) and loads (backward) in flight
+  RowLds q;
+  q.hb = 0;
+  q.lr = lr;
+  q.crs = crs;
+  q.r2s = r2s;
+  q.Tb = kTblRow * uint32_t(T);
+  // wave-uniform address of column 0 of this row and group
+  DE_GLOBAL uint8_t* const row =
+      reinterpret_cast<DE_GLOBAL uint8_t*>(gptr(F.vol) + (size_t(y) * W * Dp + g * kWave));
+  auto at = [&](int x) {
+    return reinterpret_cast<DE_GLOBAL uint16_t*>(row + size_t(x) * XB + vo);
+  };
   uint32_t t2 = 0u;
   for (int xb = xmin; xb < xmax; xb += kU) {
     uint32_t c[kU];
@@ -1152,14 +1161,14 @@ __global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
       const int x = xb + u;
       rec[u] = lr[min(x, W - 1)];
       c[u] = 0u;
-      if (x < xe && x >= xs) c[u] = score_at(smem, crs, r2s, rec[u], x - d, sh, T2);
+      if (x < xe && x >= xs) c[u] = row_score(q, rec[u], crs[x - d], r2s[x - d]);
     }
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int x = xb + u;
       if (x < xe && x >= xs) {
-        t2 = x == xs ? (c[u] << 16) : rf_step_rec(rec[u], t2, c[u]);
-        x0p[size_t(x) * Dp + lane] = hi16(t2);
+        t2 = x == xs ? (c[u] << 16) : rf_chain_z(rec[u].z, t2, __umul24(rec[u].w, c[u]) + 32768u);
+        *at(x) = uint16_t(t2 >> 16);
       }
     }
   }
@@ -1170,7 +1179,7 @@ __global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
     for (int u = 0; u < kU; ++u) {
       const int x = xt - u;
       v[u] = 0u;
-      if (x >= xs && x < xe - 1) v[u] = x0p[size_t(x) * Dp + lane];
+      if (x >= xs && x < xe - 1) v[u] = *at(x);
     }
   };
   uint32_t cur[kU], nxt[kU];
@@ -1180,10 +1189,10 @@ __global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int x = xt - u;
-      const uint4 rec = lr[max(x + 1, 0)];
+      const uint2 zw = *reinterpret_cast<const uint2*>(&lr[max(x + 1, 0)].z);
       if (x >= xs && x < xe - 1) {
-        t2 = rf_step_rec(rec, t2, cur[u]);
-        x0p[size_t(x) * Dp + lane] = hi16(t2);
+        t2 = rf_chain_z(zw.x, t2, __umul24(zw.y, cur[u]) + 32768u);
+        *at(x) = uint16_t(t2 >> 16);
       }
     }
 #pragma unroll
@@ -1192,38 +1201,55 @@ __global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
 }
 
 // V-fwd + V-bwd of one column and 64 planes, each plane on its own rectangle's
-// y range; the final values are stored back into the volume.
+// y range; the final values are stored back into the volume. Volume accesses
+// are buffer instructions with the row step as a scalar offset (as in k_col);
+// a chunk's coefficients come in one wide scalar load (the coefficient array
+// is padded for the chunks that run past a column's ends).
+template <int G>
 __global__ __launch_bounds__(64) void k_mcol(const FrameDesc* __restrict__ fp,
                                              const int4* __restrict__ rect) {
+  constexpr int Dp = G * kWave;
+  constexpr int kU = 8;   // rows in flight
   const FrameDesc F = *fp;
-  const int W = F.W, H = F.H, Dp = F.Dpad, G = F.G;
+  const int W = F.W, H = F.H;
   const int x = blockIdx.x / G, g = blockIdx.x - x * G;
   const int lane = int(threadIdx.x);
+  const uint32_t vo = 2u * uint32_t(lane);
   const int k = g * kWave + lane;
   const int4 r = k < F.D ? rect[k] : make_int4(0, 0, 0, 0);
   const bool act = k < F.D && x >= r.z && x < r.w && r.y > r.x;
   const int ys = act ? r.x : H, ye = act ? r.y : 0;
   const int ymin = wave_min(ys), ymax = wave_max(ye);
   if (ymin >= ymax) return;   // uniform over the wave
-  const size_t rs = size_t(W) * Dp;
-  uint16_t* const col = F.vol + size_t(x) * Dp + g * kWave + lane;
-  const uint32_t* ay = F.ayt + size_t(x) * H;
-  constexpr int kU = 8;   // rows in flight
+  const int ysb = W * Dp * 2;                            // bytes per volume row
+  DE_GLOBAL uint8_t* const colb =
+      reinterpret_cast<DE_GLOBAL uint8_t*>(gptr(F.vol) + (size_t(x) * Dp + g * kWave));
+  auto rs = [&](int y0) { return col_rsrc(colb + ptrdiff_t(y0) * ysb); };
+  const DE_CONST uint32_t* ay = cptr(F.ayt) + size_t(x) * H;
+  uint32_t t = 0u;
+  uint32_t cur[kU], nxt[kU], ca[kU], na[kU];
+  // rows y0 + step*u; coefficient of row y + aoff (one wide load, ascending)
   auto load_rows = [&](int y0, int step, int lo, int hi, uint32_t* v, uint32_t* a2v, int aoff) {
+    const int ylow = step > 0 ? y0 : y0 - (kU - 1);
+    const __amdgpu_buffer_rsrc_t rr = rs(ylow);
+    const DE_CONST uint32_t* const ap = ay + ylow + aoff;
+    uint32_t ab[kU];
+#pragma unroll
+    for (int i = 0; i < kU; ++i) ab[i] = ap[i];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int y = y0 + step * u;
-      a2v[u] = ay[min(max(y + aoff, 0), H - 1)];
+      const int i = step > 0 ? u : kU - 1 - u;           // index from ylow
+      a2v[u] = ab[i];
       v[u] = 0u;
-      if (y >= lo && y < hi) v[u] = col[size_t(y) * rs];
+      if (y >= lo && y < hi) v[u] = __builtin_amdgcn_raw_buffer_load_b16(rr, vo, i * ysb, 0);
     }
   };
-  uint32_t t = 0u;
-  uint32_t cur[kU], nxt[kU], ca[kU], na[kU];
   // V-fwd, rows ascending; the next chunk's rows load during this chunk's chain
   load_rows(ymin, 1, ys, ye, cur, ca, 0);
   for (int yb = ymin; yb < ymax; yb += kU) {
     if (yb + kU < ymax) load_rows(yb + kU, 1, ys, ye, nxt, na, 0);
+    const __amdgpu_buffer_rsrc_t rr = rs(yb);
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int y = yb + u;
@@ -1232,7 +1258,7 @@ __global__ __launch_bounds__(64) void k_mcol(const FrameDesc* __restrict__ fp,
           t = cur[u] << 16;
         } else {
           t = rf_chain_v(ca[u], t, rf_acc(ca[u], cur[u]));
-          col[size_t(y) * rs] = hi16(t);
+          __builtin_amdgcn_raw_buffer_store_b16(uint16_t(t >> 16), rr, vo, u * ysb, 0);
         }
       }
     }
@@ -1246,12 +1272,13 @@ __global__ __launch_bounds__(64) void k_mcol(const FrameDesc* __restrict__ fp,
   load_rows(ymax - 2, -1, ys, ye - 1, cur, ca, 1);
   for (int yt = ymax - 2; yt >= ymin; yt -= kU) {
     if (yt - kU >= ymin) load_rows(yt - kU, -1, ys, ye - 1, nxt, na, 1);
+    const __amdgpu_buffer_rsrc_t rr = rs(yt - (kU - 1));
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int y = yt - u;
       if (y >= ys && y < ye - 1) {
         t = rf_chain_v(ca[u], t, rf_acc(ca[u], cur[u]));
-        col[size_t(y) * rs] = hi16(t);
+        __builtin_amdgcn_raw_buffer_store_b16(uint16_t(t >> 16), rr, vo, (kU - 1 - u) * ysb, 0);
       }
     }
 #pragma unroll
@@ -1320,8 +1347,26 @@ hipError_t launch_masked(const FrameDesc& hf, const FrameDesc* df, const float* 
   hipLaunchKernelGGL(k_mask_dc, dim3((WH + 255) / 256), dim3(256), 0, s, prior, WH, m.dc);
   hipLaunchKernelGGL(k_mask_iv, dim3(hf.H), dim3(256), 0, s, df, m.dc, band, m.mx0, m.mx1);
   hipLaunchKernelGGL(k_mask_rect, dim3(hf.D), dim3(256), 0, s, df, m.mx0, m.mx1, m.box, m.rect);
-  hipLaunchKernelGGL(k_mrow, dim3(hf.H * hf.G), dim3(64), lds_row, s, df, m.rect);
-  hipLaunchKernelGGL(k_mcol, dim3(hf.W * hf.G), dim3(64), 0, s, df, m.rect);
+  switch (hf.G) {
+    case 1:
+      hipLaunchKernelGGL(k_mrow<1>, dim3(hf.H), dim3(64), lds_row, s, df, m.rect);
+      hipLaunchKernelGGL(k_mcol<1>, dim3(hf.W), dim3(64), 0, s, df, m.rect);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_mrow<2>, dim3(hf.H * 2), dim3(64), lds_row, s, df, m.rect);
+      hipLaunchKernelGGL(k_mcol<2>, dim3(hf.W * 2), dim3(64), 0, s, df, m.rect);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_mrow<3>, dim3(hf.H * 3), dim3(64), lds_row, s, df, m.rect);
+      hipLaunchKernelGGL(k_mcol<3>, dim3(hf.W * 3), dim3(64), 0, s, df, m.rect);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_mrow<4>, dim3(hf.H * 4), dim3(64), lds_row, s, df, m.rect);
+      hipLaunchKernelGGL(k_mcol<4>, dim3(hf.W * 4), dim3(64), 0, s, df, m.rect);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   hipLaunchKernelGGL(k_minsert, dim3((WH + 255) / 256), dim3(256), 0, s, df, m.rect, m.box,
                      m.mx0, m.mx1);
   if (lrc)
