@@ -14,7 +14,8 @@
 // the offset representation that makes one filter step two multiply-adds.
 //
 // Volume layout: v' u16 at vol[(y*W + x)*Dpad + k], Dpad = 64*G. A wave owns 64
-// consecutive k, so every volume access is one 128-byte coalesced segment.
+// consecutive k: the column pass reads and writes one 128-byte segment per row,
+// the row pass writes 8 columns at a time with 16-byte stores.
 //
 // The recurrences are sequential along their scan line, so a wave runs 64 (row
 // pass) or 64*G (column pass) independent chains; latency is hidden by
@@ -147,17 +148,18 @@ __global__ __launch_bounds__(256) void k_prep(BatchView b) {
 // contiguous bytes, so the forward stores and the backward loads one 16-byte
 // word per lane per 8 columns (one memory instruction where a column-major
 // layout needs eight, which keeps deep prefetch within the vector memory
-// counter). The backward pass's final values are transposed through a 1 KB
-// LDS tile and leave as 16-byte stores into the [y][x][Dpad] volume, lane l
-// writing disparities 8(l%8)..8(l%8)+7 of column 8b + l/8.
+// counter). The backward pass's final values are transposed through two
+// alternating 1 KB LDS tiles and leave as 16-byte stores into the [y][x][Dpad]
+// volume, lane l writing disparities 8(l%8)..8(l%8)+7 of column 8b + l/8.
 //
 // The forward scan is software-pipelined over 8-column blocks in three
 // stages: LDS loads of block n+2, score-table reads of block n+1, chain of
 // block n, so neither LDS latency is waited on where it is issued.
 //
-// The record word z = 2*L | 2a << 16 feeds the chain through op_sel (high
-// half); the intensity difference masks the low half first (v_sad_u16 would
-// add the high halves' difference too: it sums both halves).
+// The record word z = 2*L | 2a << 16 (staged in LDS as kTblRow*L | 2a << 16)
+// feeds the chain through op_sel (high half); the intensity difference masks
+// the low half first (v_sad_u16 would add the high halves' difference too: it
+// sums both halves).
 
 // 2*f' = hi16(z) * hi16(t2prev) + acc, with acc = (65536 - 2a) * c' + 32768
 // computed off the dependency chain.
@@ -333,9 +335,9 @@ __device__ __forceinline__ void row_load_blocks(uint4 (&v)[P], const RowOut& o, 
 
 // G: disparity groups of every frame in the launch; P: H-bwd prefetch depth in
 // 8-column blocks. A workgroup is 1..4 independent waves (one item each, own
-// LDS region of pw bytes): a lone wave per workgroup lets the dispatcher
-// stack several on one SIMD of a CU, while the waves of one workgroup go to
-// distinct SIMDs.
+// LDS region of pw bytes); the launcher groups waves only when every wave
+// fits in one round, where one wave per SIMD of a CU measured faster than
+// lone-wave workgroups.
 template <int G, int P>
 __global__ __launch_bounds__(256) void k_row(BatchView b, int base, int items, int pw) {
   constexpr int Dp = G * kWave;
@@ -797,8 +799,8 @@ __global__ __launch_bounds__(256) void k_census_plane(const uint8_t* __restrict_
 // Host launchers (called from the C-ABI translation unit).
 size_t row_lds_bytes(int W, int nbits, int sc_shift) {
   const int ntbl = (nbits + 1) << sc_shift;
-  // table (the row pass stages it transposed: at most 2^sc_shift rows of 132
-  // bytes; the masked row pass keeps this layout) | two row-pass transpose
+  // table (both row passes stage it transposed: at most 2^sc_shift rows of
+  // 132 bytes, or the frame's table size if larger) | two row-pass transpose
   // tiles | records, right census, intensity
   const size_t tb = std::max(size_t(ntbl) * 2, size_t(132) << sc_shift);
   return ((tb + 15) & ~size_t(15)) + 2048 + size_t(W) * (16 + 8 + 4);
@@ -811,7 +813,7 @@ hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_
   return hipGetLastError();
 }
 
-// Prefetch-depth variants (chosen per shape by the autotuner, de_tune.cpp).
+// Prefetch-depth variants (chosen per shape bucket by de_autotune).
 const int kRowVariants[kNumRowVariants] = {2, 4, 8, 16};   // H-bwd blocks of 8 in flight
 const int kColVariants[kNumColVariants] = {8, 16, 32};
 

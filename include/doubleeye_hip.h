@@ -62,7 +62,7 @@ void* de_ctx_stream(de_ctx* ctx);
 int de_ctx_set_profiling(de_ctx* ctx, int on);
 int de_ctx_stage_ms(de_ctx* ctx, double* ms4, int* runs);
 /* Launch variants of the two scan kernels (prefetch depth; row 0..3 =
- * 8/16/32/48 columns, col 0..2 = 8/16/32 rows). -1 restores the tuned/default
+ * 2/4/8/16 blocks of 8 columns, col 0..2 = 8/16/32 rows). -1 restores the tuned/default
  * choice. Results never depend on the variant. No counterpart in the
  * reference; it is the knob the autotuner turns. */
 int de_ctx_set_variants(de_ctx* ctx, int row_variant, int col_variant);
