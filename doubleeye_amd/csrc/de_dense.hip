@@ -803,7 +803,8 @@ size_t row_lds_bytes(int W, int nbits, int sc_shift) {
   // 132 bytes, or the frame's table size if larger) | two row-pass transpose
   // tiles | records, right census, intensity
   const size_t tb = std::max(size_t(ntbl) * 2, size_t(132) << sc_shift);
-  return ((tb + 15) & ~size_t(15)) + 2048 + size_t(W) * (16 + 8 + 4);
+  // (the masked row pass pads each row array by 8 entries)
+  return ((tb + 15) & ~size_t(15)) + 2048 + size_t(W + 8) * (16 + 8 + 4);
 }
 
 hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_t s) {
@@ -1096,34 +1097,109 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // Score + H-fwd + H-bwd of one row and 64 planes, each plane on its own
-// rectangle's x range.
+// rectangle's x range [xs, xe), built on the full row pass's machinery
+// (blocked forward scratch, three-stage forward pipeline, 16-byte transfers,
+// transposed tiles). The wave runs the chain over the whole span of its
+// planes' ranges in aligned 8-column blocks; a lane restarts it at its own
+// first column (f' = score there) and, scanning back, at its last (g' = f').
+// Values outside a plane's rectangle are computed and written but never read
+// (the column pass and the insert read inside the rectangles only). The LDS
+// row arrays carry 8 entries of padding past W so a block may run past the
+// row's end; volume stores of columns >= W are skipped.
+__device__ __forceinline__ void mrow_load_raw(const RowLds& q, RowRaw& R, int x0, int d) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int j = max(x0 + u - d, 0);
+    R.rec[u] = q.lr[x0 + u];
+    R.cr[u] = q.crs[j];
+    R.r2[u] = q.r2s[j];
+  }
+}
+
+__device__ __forceinline__ uint32_t mrow_chain_fwd(const RowBlk& S, uint32_t t2, int x0, int xs,
+                                                   DE_GLOBAL uint8_t* fw) {
+  uint32_t t[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const uint32_t tn = rf_chain_z(S.z[u], t2, __umul24(S.w[u], S.c[u]) + 32768u);
+    t2 = x0 + u == xs ? S.c[u] << 16 : tn;
+    t[u] = t2;
+  }
+  uint4 v;
+  v.x = __builtin_amdgcn_perm(t[1], t[0], 0x07060302u);
+  v.y = __builtin_amdgcn_perm(t[3], t[2], 0x07060302u);
+  v.z = __builtin_amdgcn_perm(t[5], t[4], 0x07060302u);
+  v.w = __builtin_amdgcn_perm(t[7], t[6], 0x07060302u);
+  *reinterpret_cast<DE_GLOBAL uint4*>(fw) = v;
+  return t2;
+}
+
+// H-bwd over P blocks b, b-1, ..., b-P+1 (as row_back_blocks), restarting at
+// column xl = xe - 1 of each lane; volume stores of columns >= W are skipped.
+template <int P, uint32_t XB>
+__device__ __forceinline__ uint32_t mrow_back_blocks(const uint4 (&v)[P], uint32_t t2,
+                                                     const uint4* lr, int b, int xl,
+                                                     const RowOut& o) {
+  uint2 zc[8], zn[8];
+  uint4 pend = make_uint4(0u, 0u, 0u, 0u);
+  row_coef8(zc, lr, 8 * b);
+  auto store = [&](int bb, const uint4& val) {
+    if (8 * bb + int(o.lane >> 3) < o.W) row_store8<XB>(o, bb, val);
+  };
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    if (k + 1 < P) row_coef8(zn, lr, 8 * (b - k - 1));
+    uint16_t* const tile = o.tile + (k & 1) * 8 * kWave;
+    const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+    const int x0 = 8 * (b - k);
+#pragma unroll
+    for (int j = 7; j >= 0; --j) {
+      const uint32_t f = (j & 1) ? (wv[j >> 1] >> 16) : (wv[j >> 1] & 0xFFFFu);
+      const uint32_t tn = rf_chain_z(zc[j].x, t2, __umul24(zc[j].y, f) + 32768u);
+      t2 = x0 + j == xl ? f << 16 : tn;
+      tile[j * kWave + o.lane] = uint16_t(t2 >> 16);
+    }
+    if (k > 0) store(b - k + 1, pend);
+    pend = *reinterpret_cast<const uint4*>(tile + o.lane * 8);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) zc[j] = zn[j];
+  }
+  store(b - P + 1, pend);
+  return t2;
+}
+
 template <int G>
 __global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
                                              const int4* __restrict__ rect) {
   constexpr int Dp = G * kWave;
   constexpr uint32_t XB = 2u * Dp;                       // bytes per volume column
-  constexpr int kU = 16;   // scores (forward) and loads (backward) in flight
+  constexpr int P = 4;                                   // H-bwd blocks in flight
   extern __shared__ __align__(16) uint8_t smem[];
   const FrameDesc F = *fp;
   const int W = F.W;
   const int y = blockIdx.x / G, g = blockIdx.x - y * G;
-  const int lane = int(threadIdx.x);
-  const uint32_t vo = 2u * uint32_t(lane);
-  const int k = g * kWave + lane;
+  const uint32_t lane = threadIdx.x & 63;
+  const int k = g * kWave + int(lane);
   const int d = F.dmin + k;
   const int4 r = k < F.D ? rect[k] : make_int4(0, 0, 0, 0);
   const bool act = k < F.D && y >= r.x && y < r.y;
   const int xs = act ? r.z : W, xe = act ? r.w : 0;
   const int xmin = wave_min(xs), xmax = wave_max(xe);
   if (xmin >= xmax) return;   // uniform over the wave
-  // LDS as the full row pass stages it: transposed score table at address 0,
-  // records with z's low half = kTblRow * L, right census, kTblRow * R
+  RowOut o;
+  o.vrow = reinterpret_cast<DE_GLOBAL uint8_t*>(gptr(F.vol) + (size_t(y) * W * Dp + g * kWave));
+  o.lane = lane;
+  o.so = (lane >> 3) * XB + (lane & 7) * 16;
+  o.W = W;
+  // LDS as the full row pass stages it, with 8 entries of padding per row array
   const int nb = (2 * F.hw + 1) * (2 * F.hh + 1) - 1;
   const int T = F.ad_T;
   const int ntt = (T + 1) * int(kTblRow / 2);
-  uint4* lr = reinterpret_cast<uint4*>(smem + ((ntt * 2 + 15) & ~15));
-  uint64_t* crs = reinterpret_cast<uint64_t*>(lr + W);
-  uint32_t* r2s = reinterpret_cast<uint32_t*>(crs + W);
+  const int Wp = W + 8;
+  o.tile = reinterpret_cast<uint16_t*>(smem + ((ntt * 2 + 15) & ~15));
+  uint4* lr = reinterpret_cast<uint4*>(o.tile + 2 * 8 * kWave);
+  uint64_t* crs = reinterpret_cast<uint64_t*>(lr + Wp);
+  uint32_t* r2s = reinterpret_cast<uint32_t*>(crs + Wp);
   {
     uint16_t* tbl = reinterpret_cast<uint16_t*>(smem);
     for (int i = lane; i < ntt; i += kWave) {
@@ -1133,12 +1209,19 @@ __global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
     const uint4* lrg = reinterpret_cast<const uint4*>(F.lrec) + size_t(y) * W;
     const uint64_t* crg = F.cr + size_t(y) * W;
     const uint8_t* Rg = F.R + size_t(y) * W;
-    for (int x = lane; x < W; x += kWave) {
-      uint4 rec = lrg[x];
-      rec.z = (rec.z & 0xFFFF0000u) | ((rec.z & 0xFFFFu) * (kTblRow / 2));
+    for (int x = lane; x < Wp; x += kWave) {
+      uint4 rec = make_uint4(0u, 0u, 0u, 65536u);
+      uint64_t cv = 0;
+      uint32_t rv = 0;
+      if (x < W) {
+        rec = lrg[x];
+        rec.z = (rec.z & 0xFFFF0000u) | ((rec.z & 0xFFFFu) * (kTblRow / 2));
+        cv = crg[x];
+        rv = kTblRow * Rg[x];
+      }
       lr[x] = rec;
-      crs[x] = crg[x];
-      r2s[x] = kTblRow * Rg[x];
+      crs[x] = cv;
+      r2s[x] = rv;
     }
   }
   __syncthreads();
@@ -1148,70 +1231,92 @@ __global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
   q.crs = crs;
   q.r2s = r2s;
   q.Tb = kTblRow * uint32_t(T);
-  // wave-uniform address of column 0 of this row and group
-  DE_GLOBAL uint8_t* const row =
-      reinterpret_cast<DE_GLOBAL uint8_t*>(gptr(F.vol) + (size_t(y) * W * Dp + g * kWave));
-  auto at = [&](int x) {
-    return reinterpret_cast<DE_GLOBAL uint16_t*>(row + size_t(x) * XB + vo);
-  };
+  const int Wb = (W + 7) >> 3;
+  o.frow = reinterpret_cast<DE_GLOBAL uint8_t*>(gptr(F.fwd)) + size_t(y * G + g) * Wb * 1024;
+  const uint32_t fo = 16u * lane;
+  const int blo = xmin >> 3, bhi = (xmax + 7) >> 3;     // blocks [blo, bhi), bhi <= Wb
+  const int nblk = bhi - blo;
+
+  // ---- H-fwd over the blocks, three stages as in k_row ----
   uint32_t t2 = 0u;
-  for (int xb = xmin; xb < xmax; xb += kU) {
-    uint32_t c[kU];
-    uint4 rec[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int x = xb + u;
-      rec[u] = lr[min(x, W - 1)];
-      c[u] = 0u;
-      if (x < xe && x >= xs) c[u] = row_score(q, rec[u], crs[x - d], r2s[x - d]);
+  int x = 8 * blo;
+  if (nblk >= 2) {
+    RowRaw R0, R1;
+    RowBlk S0, S1;
+    mrow_load_raw(q, R0, x, d);
+    mrow_load_raw(q, R1, x + 8, d);
+    row_score_raw(q, R0, S0);
+    int kb = 0;                   // S0 holds block kb scored, R1 block kb+1 loaded
+    for (; kb + 4 <= nblk; kb += 2, x += 16) {
+      mrow_load_raw(q, R0, x + 16, d);
+      row_score_raw(q, R1, S1);
+      t2 = mrow_chain_fwd(S0, t2, x, xs, o.frow + size_t(x >> 3) * 1024 + fo);
+      mrow_load_raw(q, R1, x + 24, d);
+      row_score_raw(q, R0, S0);
+      t2 = mrow_chain_fwd(S1, t2, x + 8, xs, o.frow + size_t((x + 8) >> 3) * 1024 + fo);
     }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int x = xb + u;
-      if (x < xe && x >= xs) {
-        t2 = x == xs ? (c[u] << 16) : rf_chain_z(rec[u].z, t2, __umul24(rec[u].w, c[u]) + 32768u);
-        *at(x) = uint16_t(t2 >> 16);
-      }
+    if (nblk - kb == 3) {
+      mrow_load_raw(q, R0, x + 16, d);
+      row_score_raw(q, R1, S1);
+      t2 = mrow_chain_fwd(S0, t2, x, xs, o.frow + size_t(x >> 3) * 1024 + fo);
+      row_score_raw(q, R0, S0);
+      t2 = mrow_chain_fwd(S1, t2, x + 8, xs, o.frow + size_t((x + 8) >> 3) * 1024 + fo);
+      t2 = mrow_chain_fwd(S0, t2, x + 16, xs, o.frow + size_t((x + 16) >> 3) * 1024 + fo);
+    } else {
+      row_score_raw(q, R1, S1);
+      t2 = mrow_chain_fwd(S0, t2, x, xs, o.frow + size_t(x >> 3) * 1024 + fo);
+      t2 = mrow_chain_fwd(S1, t2, x + 8, xs, o.frow + size_t((x + 8) >> 3) * 1024 + fo);
+    }
+  } else {
+    RowRaw R0;
+    RowBlk S0;
+    mrow_load_raw(q, R0, x, d);
+    row_score_raw(q, R0, S0);
+    t2 = mrow_chain_fwd(S0, t2, x, xs, o.frow + size_t(x >> 3) * 1024 + fo);
+  }
+
+  // ---- H-bwd over the blocks, descending; a lane restarts at xe - 1 ----
+  const int xl = xe - 1;
+  int bb = bhi - 1;
+  if (bb - P + 1 >= blo) {
+    uint4 A[P], B[P];
+    row_load_blocks<P>(A, o, bb);
+    for (; bb - 3 * P + 1 >= blo; bb -= 2 * P) {
+      row_load_blocks<P>(B, o, bb - P);
+      t2 = mrow_back_blocks<P, XB>(A, t2, lr, bb, xl, o);
+      row_load_blocks<P>(A, o, bb - 2 * P);
+      t2 = mrow_back_blocks<P, XB>(B, t2, lr, bb - P, xl, o);
+    }
+    if (bb - 2 * P + 1 >= blo) {
+      row_load_blocks<P>(B, o, bb - P);
+      t2 = mrow_back_blocks<P, XB>(A, t2, lr, bb, xl, o);
+      t2 = mrow_back_blocks<P, XB>(B, t2, lr, bb - P, xl, o);
+      bb -= 2 * P;
+    } else {
+      t2 = mrow_back_blocks<P, XB>(A, t2, lr, bb, xl, o);
+      bb -= P;
     }
   }
-  // backward: the forward values of the next chunk are loaded while the
-  // current chunk runs its chain (the chunks touch disjoint columns)
-  auto load_bwd = [&](int xt, uint32_t* v) {
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int x = xt - u;
-      v[u] = 0u;
-      if (x >= xs && x < xe - 1) v[u] = *at(x);
-    }
-  };
-  uint32_t cur[kU], nxt[kU];
-  load_bwd(xmax - 2, cur);
-  for (int xt = xmax - 2; xt >= xmin; xt -= kU) {
-    if (xt - kU >= xmin) load_bwd(xt - kU, nxt);
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int x = xt - u;
-      const uint2 zw = *reinterpret_cast<const uint2*>(&lr[max(x + 1, 0)].z);
-      if (x >= xs && x < xe - 1) {
-        t2 = rf_chain_z(zw.x, t2, __umul24(zw.y, cur[u]) + 32768u);
-        *at(x) = uint16_t(t2 >> 16);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) cur[u] = nxt[u];
+  for (; bb >= blo; --bb) {
+    uint4 v[1];
+    row_load_blocks<1>(v, o, bb);
+    t2 = mrow_back_blocks<1, XB>(v, t2, lr, bb, xl, o);
   }
 }
 
 // V-fwd + V-bwd of one column and 64 planes, each plane on its own rectangle's
-// y range; the final values are stored back into the volume. Volume accesses
-// are buffer instructions with the row step as a scalar offset (as in k_col);
-// a chunk's coefficients come in one wide scalar load (the coefficient array
-// is padded for the chunks that run past a column's ends).
+// y range [ys, ye); the final values are stored back into the volume. As in
+// the masked row pass, the chains run over the wave's whole row span and a
+// lane restarts at its own first row (forward) and last row (backward); values
+// outside a plane's rectangle are never read. The scans are the full column
+// pass's: buffer instructions with the row step as a scalar offset, one wide
+// scalar coefficient load per chunk (the coefficient array is padded for the
+// chunks that run past a column's ends), two alternating prefetch sets.
 template <int G>
 __global__ __launch_bounds__(64) void k_mcol(const FrameDesc* __restrict__ fp,
                                              const int4* __restrict__ rect) {
   constexpr int Dp = G * kWave;
-  constexpr int kU = 8;   // rows in flight
+  constexpr int U = 8;   // rows in flight
   const FrameDesc F = *fp;
   const int W = F.W, H = F.H;
   const int x = blockIdx.x / G, g = blockIdx.x - x * G;
@@ -1229,64 +1334,90 @@ __global__ __launch_bounds__(64) void k_mcol(const FrameDesc* __restrict__ fp,
   auto rs = [&](int y0) { return col_rsrc(colb + ptrdiff_t(y0) * ysb); };
   const DE_CONST uint32_t* ay = cptr(F.ayt) + size_t(x) * H;
   uint32_t t = 0u;
-  uint32_t cur[kU], nxt[kU], ca[kU], na[kU];
-  // rows y0 + step*u; coefficient of row y + aoff (one wide load, ascending)
-  auto load_rows = [&](int y0, int step, int lo, int hi, uint32_t* v, uint32_t* a2v, int aoff) {
-    const int ylow = step > 0 ? y0 : y0 - (kU - 1);
-    const __amdgpu_buffer_rsrc_t rr = rs(ylow);
-    const DE_CONST uint32_t* const ap = ay + ylow + aoff;
-    uint32_t ab[kU];
+
+  // ---- V-fwd over rows [ymin, ymax), restart at ys ----
+  {
+    auto ldf = [&](uint32_t (&v)[U], int y0) {
+      const __amdgpu_buffer_rsrc_t rr = rs(y0);
 #pragma unroll
-    for (int i = 0; i < kU; ++i) ab[i] = ap[i];
+      for (int u = 0; u < U; ++u) v[u] = __builtin_amdgcn_raw_buffer_load_b16(rr, vo, u * ysb, 0);
+    };
+    auto fwd = [&](const uint32_t (&v)[U], int y0, int n) {
+      const __amdgpu_buffer_rsrc_t rr = rs(y0);
+      const DE_CONST uint32_t* const ap = ay + y0;
+      uint32_t ab[U];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int y = y0 + step * u;
-      const int i = step > 0 ? u : kU - 1 - u;           // index from ylow
-      a2v[u] = ab[i];
-      v[u] = 0u;
-      if (y >= lo && y < hi) v[u] = __builtin_amdgcn_raw_buffer_load_b16(rr, vo, i * ysb, 0);
-    }
-  };
-  // V-fwd, rows ascending; the next chunk's rows load during this chunk's chain
-  load_rows(ymin, 1, ys, ye, cur, ca, 0);
-  for (int yb = ymin; yb < ymax; yb += kU) {
-    if (yb + kU < ymax) load_rows(yb + kU, 1, ys, ye, nxt, na, 0);
-    const __amdgpu_buffer_rsrc_t rr = rs(yb);
+      for (int u = 0; u < U; ++u) ab[u] = ap[u];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int y = yb + u;
-      if (y >= ys && y < ye) {
-        if (y == ys) {
-          t = cur[u] << 16;
-        } else {
-          t = rf_chain_v(ca[u], t, rf_acc(ca[u], cur[u]));
+      for (int u = 0; u < U; ++u) {
+        if (u < n) {
+          const uint32_t tn = rf_chain_v(ab[u], t, rf_acc(ab[u], v[u]));
+          t = y0 + u == ys ? v[u] << 16 : tn;
           __builtin_amdgcn_raw_buffer_store_b16(uint16_t(t >> 16), rr, vo, u * ysb, 0);
         }
       }
+    };
+    uint32_t A[U], B[U];
+    int y = ymin;
+    ldf(A, y);
+    for (; y + 2 * U <= ymax; y += 2 * U) {
+      ldf(B, y + U);
+      fwd(A, y, U);
+      ldf(A, y + 2 * U);
+      fwd(B, y + U, U);
     }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      cur[u] = nxt[u];
-      ca[u] = na[u];
+    if (y + U <= ymax) {
+      ldf(B, y + U);
+      fwd(A, y, U);
+      fwd(B, y + U, ymax - y - U);
+    } else {
+      fwd(A, y, ymax - y);
     }
   }
-  // V-bwd, rows descending, coefficient of y + 1
-  load_rows(ymax - 2, -1, ys, ye - 1, cur, ca, 1);
-  for (int yt = ymax - 2; yt >= ymin; yt -= kU) {
-    if (yt - kU >= ymin) load_rows(yt - kU, -1, ys, ye - 1, nxt, na, 1);
-    const __amdgpu_buffer_rsrc_t rr = rs(yt - (kU - 1));
+
+  // ---- V-bwd over rows ymax-1 .. ymin, coefficient of y+1, restart at ye-1 ----
+  {
+    auto ldb = [&](uint32_t (&v)[U], int ytop) {      // rows ytop, ytop-1, ..., ytop-U+1
+      const __amdgpu_buffer_rsrc_t rr = rs(ytop - U + 1);
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int y = yt - u;
-      if (y >= ys && y < ye - 1) {
-        t = rf_chain_v(ca[u], t, rf_acc(ca[u], cur[u]));
-        __builtin_amdgcn_raw_buffer_store_b16(uint16_t(t >> 16), rr, vo, (kU - 1 - u) * ysb, 0);
+      for (int u = 0; u < U; ++u)
+        v[u] = __builtin_amdgcn_raw_buffer_load_b16(rr, vo, (U - 1 - u) * ysb, 0);
+    };
+    const int yl = ye - 1;
+    auto bwd = [&](const uint32_t (&v)[U], int ytop, int n) {
+      const __amdgpu_buffer_rsrc_t rr = rs(ytop - U + 1);
+      const DE_CONST uint32_t* const ap = ay + (ytop - U + 2);
+      uint32_t ab[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) ab[u] = ap[u];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (u < n) {
+          const uint32_t a2 = ab[U - 1 - u];
+          const uint32_t tn = rf_chain_v(a2, t, rf_acc(a2, v[u]));
+          t = ytop - u == yl ? v[u] << 16 : tn;
+          __builtin_amdgcn_raw_buffer_store_b16(uint16_t(t >> 16), rr, vo, (U - 1 - u) * ysb,
+                                                0);
+        }
       }
+    };
+    uint32_t A[U], B[U];
+    int y = ymax - 1;                                    // next row to process
+    const int nrow = ymax - ymin;
+    int done = 0;
+    ldb(A, y);
+    for (; done + 2 * U <= nrow; done += 2 * U, y -= 2 * U) {
+      ldb(B, y - U);
+      bwd(A, y, U);
+      ldb(A, y - 2 * U);
+      bwd(B, y - U, U);
     }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      cur[u] = nxt[u];
-      ca[u] = na[u];
+    if (done + U <= nrow) {
+      ldb(B, y - U);
+      bwd(A, y, U);
+      bwd(B, y - U, nrow - done - U);
+    } else {
+      bwd(A, y, nrow - done);
     }
   }
 }
