@@ -217,9 +217,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
   // the pixel owner then updates its beta from that. Item indices are unique
   // in a bucket, so excluding by index equals excluding by position, and the
   // float expressions are the reference's, operation for operation.
-  float* const bt1 = S.beta;                      // [W] bucket best rho
-  float* const bt2 = S.beta + W;                  // [W] bucket runner-up
-  int* const bti = S.cursor;                      // [W] item index of the best (-1: empty)
+  //
+  // The three phases run branch-free: absent candidates, pixels at or beyond
+  // W and unused bucket slots compute on neutral operands (slot 0, kNone) and
+  // their results are dropped by selects, so every lane issues the same
+  // instructions. A push of kNone leaves a top-2 as it is.
+  float2* const bt = reinterpret_cast<float2*>(S.beta);   // [W] bucket best, runner-up rho
+  int* const bti = S.cursor;                              // [W] item index of the best (-1: empty)
   float c_s[kPerThread][2], c_b[kPerThread][2], c_r[kPerThread][2];
   int c_xr[kPerThread][2], c_n[kPerThread];
   int b_lo[kPerThread], b_cnt[kPerThread], b_it[kPerThread][kBucketRegs];
@@ -254,57 +258,64 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
     }
   }
   for (int it = 0; it < F.iters; ++it) {
-    // rho of every pixel from its own betas
+    // rho of every pixel from its own betas. Top2::push of beta 0 then beta 1
+    // gives excl(0) = beta 1 (if present and above kNone, else kNone) and
+    // excl(1) = beta 0 (if above kNone, else kNone).
 #pragma unroll
     for (int k = 0; k < kPerThread; ++k) {
       const int n = c_n[k];
-      if (!n) continue;
       const int x = tid + k * nt;
-      Top2 t;
-      t.push(c_b[k][0], 0);
-      if (n > 1) t.push(c_b[k][1], 1);
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        if (j < n) {
-          const float tgt = c_s[k][j] - fmax_ref(lambda, t.excl(j));
-          c_r[k][j] = keep * tgt + damping * c_r[k][j];
-          S.rho[2 * x + j] = c_r[k][j];
-        }
-      }
+      const float e0 = (n > 1 && c_b[k][1] > kNone) ? c_b[k][1] : kNone;
+      const float e1 = (c_b[k][0] > kNone) ? c_b[k][0] : kNone;
+      const float t0 = c_s[k][0] - fmax_ref(lambda, e0);
+      const float t1 = c_s[k][1] - fmax_ref(lambda, e1);
+      const float r0 = keep * t0 + damping * c_r[k][0];
+      const float r1 = keep * t1 + damping * c_r[k][1];
+      c_r[k][0] = n > 0 ? r0 : c_r[k][0];
+      c_r[k][1] = n > 1 ? r1 : c_r[k][1];
+      if (x < W) reinterpret_cast<float2*>(S.rho)[x] = make_float2(c_r[k][0], c_r[k][1]);
     }
     __syncthreads();
-    // each bucket's top-2 of rho
+    // each bucket's top-2 of rho (slots past the count push kNone)
 #pragma unroll
     for (int k = 0; k < kPerThread; ++k) {
       const int xr = tid + k * nt;
-      if (xr >= W) continue;
       const int cnt = b_cnt[k];
-      Top2 t;
+      float t1 = kNone, t2 = kNone;
+      int ti = -1;
+      auto push = [&](float v, int i) {
+        const bool g1 = v > t1, g2 = v > t2;
+        t2 = g1 ? t1 : (g2 ? v : t2);
+        t1 = g1 ? v : t1;
+        ti = g1 ? i : ti;
+      };
 #pragma unroll
-      for (int i = 0; i < kBucketRegs; ++i)
-        if (i < cnt) t.push(S.rho[b_it[k][i]], b_it[k][i]);
+      for (int i = 0; i < kBucketRegs; ++i) {
+        const float rv = S.rho[b_it[k][i]];
+        push(i < cnt ? rv : kNone, b_it[k][i]);
+      }
       for (int i = kBucketRegs; i < cnt; ++i) {
         const int idx = S.items[b_lo[k] + i];
-        t.push(S.rho[idx], idx);
+        push(S.rho[idx], idx);
       }
-      bt1[xr] = t.b1;
-      bt2[xr] = t.b2;
-      bti[xr] = t.i1;
+      if (xr < W) {
+        bt[xr] = make_float2(t1, t2);
+        bti[xr] = ti;
+      }
     }
     __syncthreads();
     // beta of every bucketed candidate from its bucket's top-2
 #pragma unroll
     for (int k = 0; k < kPerThread; ++k) {
-      const int n = c_n[k];
-      if (!n) continue;
       const int x = tid + k * nt;
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const int xr = c_xr[k][j];
-        if (j < n && xr >= 0) {
-          const float ex = bti[xr] == 2 * x + j ? bt2[xr] : bt1[xr];
-          c_b[k][j] = keep * (c_s[k][j] - fmax_ref(gamma, ex)) + damping * c_b[k][j];
-        }
+        const int xr = c_xr[k][j];          // -1: absent or outside the row
+        const int xq = max(xr, 0);
+        const float2 tb = bt[xq];
+        const float ex = bti[xq] == 2 * x + j ? tb.y : tb.x;
+        const float nb = keep * (c_s[k][j] - fmax_ref(gamma, ex)) + damping * c_b[k][j];
+        c_b[k][j] = xr >= 0 ? nb : c_b[k][j];
       }
     }
   }
