@@ -42,31 +42,16 @@ __device__ __forceinline__ uint32_t ordered(float v) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-struct Top2 {
-  float b1 = kNone, b2 = kNone;
-  int i1 = -1;
-  __device__ void push(float v, int i) {
-    if (v > b1) { b2 = b1; b1 = v; i1 = i; }
-    else if (v > b2) { b2 = v; }
-  }
-  __device__ float excl(int i) const { return i == i1 ? b2 : b1; }
-};
-
 constexpr int kBucketRegs = 4;                      // bucket items kept in registers
 
 // LDS layout for one row of width W (bytes): see solve_lds_bytes().
 struct SolveSmem {
-  float* cs;        // [2W]
-  int16_t* cd;      // [2W]
-  uint8_t* cn;      // [W]
-  float* beta;      // [2W]   (reused as claim keys after the iterations)
-  float* rho;       // [2W]
+  float* rho;       // [2W]  rho per candidate (2x + j); claim keys [W] (u64) after the iterations
+  float2* bt;       // [W]   bucket best and runner-up rho
+  unsigned long long* revk;   // [W] reverse-match key per right pixel (0: no claimant)
   int* bstart;      // [W + 1]
-  int* cursor;      // [W]
+  int* cursor;      // [W]   placement cursors; item index of each bucket's best during the iterations
   int16_t* items;   // [2W]
-  int16_t* rev;     // [W]
-  float* best;      // [W]
-  int8_t* bestj;    // [W]
   int* scan;        // [kSolveThreads]
 };
 
@@ -74,18 +59,13 @@ __device__ SolveSmem carve(uint8_t* base, int W) {
   SolveSmem s;
   size_t o = 0;
   auto take = [&](size_t bytes) { uint8_t* p = base + o; o += (bytes + 15) & ~size_t(15); return p; };
-  s.cs = reinterpret_cast<float*>(take(8 * size_t(W)));
-  s.beta = reinterpret_cast<float*>(take(8 * size_t(W)));
   s.rho = reinterpret_cast<float*>(take(8 * size_t(W)));
+  s.bt = reinterpret_cast<float2*>(take(8 * size_t(W)));
+  s.revk = reinterpret_cast<unsigned long long*>(take(8 * size_t(W)));
   s.bstart = reinterpret_cast<int*>(take(4 * size_t(W + 1)));
   s.cursor = reinterpret_cast<int*>(take(4 * size_t(W)));
-  s.best = reinterpret_cast<float*>(take(4 * size_t(W)));
-  s.scan = reinterpret_cast<int*>(take(4 * size_t(kSolveThreads)));
-  s.cd = reinterpret_cast<int16_t*>(take(4 * size_t(W)));
   s.items = reinterpret_cast<int16_t*>(take(4 * size_t(W)));
-  s.rev = reinterpret_cast<int16_t*>(take(2 * size_t(W)));
-  s.cn = take(size_t(W));
-  s.bestj = reinterpret_cast<int8_t*>(take(size_t(W)));
+  s.scan = reinterpret_cast<int*>(take(4 * size_t(kSolveThreads)));
   return s;
 }
 
@@ -111,10 +91,12 @@ __device__ float confidence_dev(float s1, float s2, float lambda, float lrc, flo
 }  // namespace
 
 // KP: pixels (and buckets) per thread, at least ceil(W / kSolveThreads) for
-// every frame of the launch.
+// every frame of the launch. Thread tid owns pixels x = tid + k*nt and the
+// buckets (right pixels) of the same indices, k < KP; everything about its
+// own pixels stays in its registers from the candidate parse to the outputs.
 template <int KP>
-__global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
-  constexpr int kPerThread = KP;
+__global__ __launch_bounds__(kSolveThreads) __attribute__((amdgpu_waves_per_eu(KP <= 4 ? 4 : 2)))
+void k_solve(BatchView b) {
   extern __shared__ __align__(16) uint8_t smem_raw[];
   const int item = blockIdx.x;
   const int f = find_frame(b, kStageSolve, item);
@@ -130,32 +112,62 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
   DE_GLOBAL float* g_margin = gptr(F.margin);
   const size_t row = size_t(y) * W;
 
-  // --- candidates, margin, bucket counts ---
   for (int x = tid; x < W + 1; x += nt) S.bstart[x] = 0;
+  for (int x = tid; x < W; x += nt) S.revk[x] = 0ull;
   __syncthreads();
-  for (int x = tid; x < W; x += nt) {
-    const uint4 r = recs[x];
-    int n = 0;
-    S.cs[2 * x] = S.cs[2 * x + 1] = kNone;
-    S.cd[2 * x] = S.cd[2 * x + 1] = -1;
-    if (r.x) {
-      S.cs[2 * x] = key_score(r.x); S.cd[2 * x] = int16_t(key_k(r.x)); n = 1;
-      if (r.y) { S.cs[2 * x + 1] = key_score(r.y); S.cd[2 * x + 1] = int16_t(key_k(r.y)); n = 2; }
+
+  // --- candidates and margin into registers; bucket counts; reverse match ---
+  // A candidate j of pixel x claims right pixel xr = x - (dmin + k_j) when it
+  // lies in the row. The reverse match of xr is its claimant of highest score,
+  // the smallest x among equal scores (scores are never -0, so the ordered
+  // bits compare as the floats do): one 64-bit atomic max per claim over
+  // (score, 0xFFFF - x, k). x is unique within a bucket (a pixel's two
+  // candidates have different k).
+  // The margin, (n == 0) ? 0 : s1 - max(s2, lambda) with lambda for a missing s2.
+  auto margin_of = [&](int n, float s1, float s2) {
+    const float alt = (n < 2) ? lambda : fmax_ref(s2, lambda);
+    return (n == 0) ? 0.f : s1 - alt;
+  };
+  float c_s[KP][2], c_b[KP][2], c_r[KP][2];
+  int c_xr[KP][2], c_n[KP];
+  uint32_t c_kk[KP];            // k of candidate 0 | k of candidate 1 << 8
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    const int x = tid + k * nt;
+    c_n[k] = 0; c_kk[k] = 0u;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      c_s[k][j] = kNone; c_b[k][j] = 0.f; c_r[k][j] = 0.f; c_xr[k][j] = -1;
     }
-    S.cn[x] = uint8_t(n);
-    const float alt = (n < 2) ? lambda : fmax_ref(S.cs[2 * x + 1], lambda);
-    g_margin[row + x] = (n == 0) ? 0.f : S.cs[2 * x] - alt;
-    S.beta[2 * x] = S.beta[2 * x + 1] = 0.f;
-    S.rho[2 * x] = S.rho[2 * x + 1] = 0.f;
-    for (int j = 0; j < n; ++j) {
-      const int xr = x - (dmin + S.cd[2 * x + j]);
-      if (xr >= 0 && xr < W) atomicAdd(&S.bstart[xr], 1);
+    if (x < W) {
+      const uint4 r = recs[x];
+      int n = 0;
+      if (r.x) {
+        c_s[k][0] = key_score(r.x); c_kk[k] = key_k(r.x); n = 1;
+        if (r.y) { c_s[k][1] = key_score(r.y); c_kk[k] |= key_k(r.y) << 8; n = 2; }
+      }
+      c_n[k] = n;
+      g_margin[row + x] = margin_of(n, c_s[k][0], c_s[k][1]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const uint32_t kj = (c_kk[k] >> (8 * j)) & 0xFFu;
+        const int xr = x - (dmin + int(kj));
+        if (j < n && xr >= 0 && xr < W) {
+          c_xr[k][j] = xr;
+          atomicAdd(&S.bstart[xr], 1);
+          const unsigned long long key =
+              (static_cast<unsigned long long>(ordered(c_s[k][j])) << 32) |
+              (uint32_t(0xFFFF - x) << 16) | kj;
+          atomicMax(&S.revk[xr], key);
+        }
+      }
     }
   }
   __syncthreads();
 
-  // --- exclusive scan of the counts: chunk per thread, inclusive scan within
-  // each wave through lane shuffles, then over the wave totals ---
+  // --- exclusive scan of the counts (into bstart and the cursors): chunk per
+  // thread, inclusive scan within each wave through lane shuffles, then over
+  // the wave totals ---
   {
     const int per = (W + nt - 1) / nt;
     const int lo = min(W, tid * per), hi = min(W, lo + per);
@@ -181,37 +193,27 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
     }
     __syncthreads();
     int run = (wave > 0 ? S.scan[wave - 1] : 0) + v - sum;
-    for (int i = lo; i < hi; ++i) { const int c = S.bstart[i]; S.bstart[i] = run; run += c; }
+    for (int i = lo; i < hi; ++i) {
+      const int c = S.bstart[i];
+      S.bstart[i] = run;
+      S.cursor[i] = run;
+      run += c;
+    }
     if (tid == nt - 1) S.bstart[W] = S.scan[nwave - 1];
   }
   __syncthreads();
-  for (int x = tid; x < W; x += nt) S.cursor[x] = S.bstart[x];
-  __syncthreads();
-  for (int x = tid; x < W; x += nt) {
-    const int n = S.cn[x];
-    for (int j = 0; j < n; ++j) {
-      const int xr = x - (dmin + S.cd[2 * x + j]);
-      if (xr >= 0 && xr < W) S.items[atomicAdd(&S.cursor[xr], 1)] = int16_t(2 * x + j);
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int xr = c_xr[k][j];
+      if (xr >= 0) S.items[atomicAdd(&S.cursor[xr], 1)] = int16_t(2 * (tid + k * nt) + j);
     }
   }
   __syncthreads();
-
-  // --- reverse match: best claimant by score, first (smallest x) of equals ---
-  for (int xr = tid; xr < W; xr += nt) {
-    float bv = kNone;
-    int bx = 0x7FFFFFFF, bd = -1;
-    for (int p = S.bstart[xr]; p < S.bstart[xr + 1]; ++p) {
-      const int idx = S.items[p];
-      const float v = S.cs[idx];
-      const int xx = idx >> 1;
-      if (v > bv || (v == bv && xx < bx)) { bv = v; bx = xx; bd = S.cd[idx]; }
-    }
-    S.rev[xr] = int16_t(bd);
-  }
 
   // --- message passing ---
-  // Thread tid owns pixels x = tid + k*nt and buckets xr = tid + k*nt. A
-  // pixel's candidates, beta and rho live in its owner's registers across
+  // A pixel's candidates, beta and rho live in its owner's registers across
   // the iterations; rho goes to LDS for the bucket owners, which publish each
   // bucket's top-2 of rho (best, runner-up, item index of the first best);
   // the pixel owner then updates its beta from that. Item indices are unique
@@ -222,47 +224,28 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
   // W and unused bucket slots compute on neutral operands (slot 0, kNone) and
   // their results are dropped by selects, so every lane issues the same
   // instructions. A push of kNone leaves a top-2 as it is.
-  float2* const bt = reinterpret_cast<float2*>(S.beta);   // [W] bucket best, runner-up rho
-  int* const bti = S.cursor;                              // [W] item index of the best (-1: empty)
-  float c_s[kPerThread][2], c_b[kPerThread][2], c_r[kPerThread][2];
-  int c_xr[kPerThread][2], c_n[kPerThread];
-  int b_lo[kPerThread], b_cnt[kPerThread], b_it[kPerThread][kBucketRegs];
+  int* const bti = S.cursor;
+  int b_cnt[KP], b_it[KP][kBucketRegs];
 #pragma unroll
-  for (int k = 0; k < kPerThread; ++k) {
-    const int x = tid + k * nt;
-    c_n[k] = 0;
+  for (int k = 0; k < KP; ++k) {
+    const int xr = tid + k * nt;
     b_cnt[k] = 0;
-    b_lo[k] = 0;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      c_s[k][j] = kNone; c_b[k][j] = 0.f; c_r[k][j] = 0.f; c_xr[k][j] = -1;
-    }
 #pragma unroll
     for (int i = 0; i < kBucketRegs; ++i) b_it[k][i] = 0;
-    if (x < W) {
-      const int n = S.cn[x];
-      c_n[k] = n;
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        if (j < n) {
-          c_s[k][j] = S.cs[2 * x + j];
-          const int xr = x - (dmin + S.cd[2 * x + j]);
-          c_xr[k][j] = (xr >= 0 && xr < W) ? xr : -1;
-        }
-      }
-      const int lo = S.bstart[x], cnt = S.bstart[x + 1] - lo;
-      b_lo[k] = lo;
+    if (xr < W) {
+      const int lo = S.bstart[xr], cnt = S.bstart[xr + 1] - lo;
       b_cnt[k] = cnt;
 #pragma unroll
       for (int i = 0; i < kBucketRegs; ++i) if (i < cnt) b_it[k][i] = S.items[lo + i];
     }
   }
   for (int it = 0; it < F.iters; ++it) {
-    // rho of every pixel from its own betas. Top2::push of beta 0 then beta 1
-    // gives excl(0) = beta 1 (if present and above kNone, else kNone) and
+    // rho of every pixel from its own betas. The reference's top-2 (strictly
+    // greater displaces, from kNone) over beta 0 then beta 1 excludes to
+    // excl(0) = beta 1 (if present and above kNone, else kNone) and
     // excl(1) = beta 0 (if above kNone, else kNone).
 #pragma unroll
-    for (int k = 0; k < kPerThread; ++k) {
+    for (int k = 0; k < KP; ++k) {
       const int n = c_n[k];
       const int x = tid + k * nt;
       const float e0 = (n > 1 && c_b[k][1] > kNone) ? c_b[k][1] : kNone;
@@ -278,7 +261,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
     __syncthreads();
     // each bucket's top-2 of rho (slots past the count push kNone)
 #pragma unroll
-    for (int k = 0; k < kPerThread; ++k) {
+    for (int k = 0; k < KP; ++k) {
       const int xr = tid + k * nt;
       const int cnt = b_cnt[k];
       float t1 = kNone, t2 = kNone;
@@ -294,89 +277,92 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
         const float rv = S.rho[b_it[k][i]];
         push(i < cnt ? rv : kNone, b_it[k][i]);
       }
-      for (int i = kBucketRegs; i < cnt; ++i) {
-        const int idx = S.items[b_lo[k] + i];
-        push(S.rho[idx], idx);
+      if (cnt > kBucketRegs) {
+        const int lo = S.bstart[xr];
+        for (int i = kBucketRegs; i < cnt; ++i) {
+          const int idx = S.items[lo + i];
+          push(S.rho[idx], idx);
+        }
       }
       if (xr < W) {
-        bt[xr] = make_float2(t1, t2);
+        S.bt[xr] = make_float2(t1, t2);
         bti[xr] = ti;
       }
     }
     __syncthreads();
     // beta of every bucketed candidate from its bucket's top-2
 #pragma unroll
-    for (int k = 0; k < kPerThread; ++k) {
+    for (int k = 0; k < KP; ++k) {
       const int x = tid + k * nt;
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int xr = c_xr[k][j];          // -1: absent or outside the row
         const int xq = max(xr, 0);
-        const float2 tb = bt[xq];
+        const float2 tb = S.bt[xq];
         const float ex = bti[xq] == 2 * x + j ? tb.y : tb.x;
         const float nb = keep * (c_s[k][j] - fmax_ref(gamma, ex)) + damping * c_b[k][j];
         c_b[k][j] = xr >= 0 ? nb : c_b[k][j];
       }
     }
   }
-  // final betas and rhos back to LDS for the belief pass (the bucket top-2
-  // arrays share the beta storage: every thread is past its last read)
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kPerThread; ++k) {
-    const int x = tid + k * nt;
-    if (x >= W) continue;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      S.beta[2 * x + j] = c_b[k][j];
-      S.rho[2 * x + j] = c_r[k][j];
-    }
-  }
-  __syncthreads();
 
-  // --- belief, gates, claim ---
-  for (int x = tid; x < W; x += nt) {
+  // --- belief and gates (registers), then the claim: each target right pixel
+  // goes to its claimant of highest belief, the smallest x among equals. The
+  // claim keys take the rho storage, last read before the final barrier of
+  // the iterations (or never, without iterations). ---
+  unsigned long long* claim = reinterpret_cast<unsigned long long*>(S.rho);
+  for (int x = tid; x < W; x += nt) claim[x] = 0ull;
+  float c_best[KP];
+  int c_bj[KP];
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    const int x = tid + k * nt;
+    const int n = c_n[k];
     float bst = kNone;
     int bj = -1;
-    for (int j = 0; j < S.cn[x]; ++j) {
-      const float c = S.cs[2 * x + j];
-      if (c <= lambda) continue;
-      const float bel = (F.iters > 0) ? (S.beta[2 * x + j] + S.rho[2 * x + j] - c) : c;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const float c = c_s[k][j];
+      if (j >= n || c <= lambda) continue;
+      const float bel = (F.iters > 0) ? (c_b[k][j] + c_r[k][j] - c) : c;
       if (bel > bst) { bst = bel; bj = j; }
     }
-    if (bj >= 0 && F.min_margin > 0.f && g_margin[row + x] < F.min_margin) bj = -1;
+    if (bj >= 0 && F.min_margin > 0.f && margin_of(n, c_s[k][0], c_s[k][1]) < F.min_margin)
+      bj = -1;
     if (bj >= 0 && F.min_ratio > 0.f) {
-      const float s2 = (S.cn[x] > 1) ? fmax_ref(S.cs[2 * x + 1], lambda) : lambda;
-      const float c1 = fmax_ref(1.f - S.cs[2 * x], 1e-6f);
+      const float s2 = (n > 1) ? fmax_ref(c_s[k][1], lambda) : lambda;
+      const float c1 = fmax_ref(1.f - c_s[k][0], 1e-6f);
       if ((1.f - s2) / c1 < F.min_ratio) bj = -1;
     }
-    if (bj >= 0 && x - (dmin + S.cd[2 * x + bj]) < 0) bj = -1;
-    S.best[x] = bst;
-    S.bestj[x] = int8_t(bj);
+    const int kb = int((c_kk[k] >> (bj == 1 ? 8 : 0)) & 0xFFu);
+    if (bj >= 0 && x - (dmin + kb) < 0) bj = -1;
+    c_best[k] = bst;
+    c_bj[k] = (x < W) ? bj : -1;
   }
   __syncthreads();
-  unsigned long long* claim = reinterpret_cast<unsigned long long*>(S.beta);
-  for (int x = tid; x < W; x += nt) claim[x] = 0ull;
-  __syncthreads();
-  for (int x = tid; x < W; x += nt) {
-    const int bj = S.bestj[x];
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    const int bj = c_bj[k];
     if (bj < 0) continue;
-    const int xr = x - (dmin + S.cd[2 * x + bj]);
+    const int x = tid + k * nt;
+    const int xr = x - (dmin + int((c_kk[k] >> (8 * bj)) & 0xFFu));
     const unsigned long long key =
-        (static_cast<unsigned long long>(ordered(S.best[x])) << 32) | (0xFFFFFFFFu - uint32_t(x));
+        (static_cast<unsigned long long>(ordered(c_best[k])) << 32) | (0xFFFFFFFFu - uint32_t(x));
     atomicMax(&claim[xr], key);
   }
   __syncthreads();
 
   // --- outputs ---
-  const float q = 1.f / float(kOne);
   const size_t WH = size_t(W) * F.H;
-  for (int x = tid; x < W; x += nt) {
-    const int bj = S.bestj[x];
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    const int x = tid + k * nt;
+    if (x >= W) continue;
+    const int bj = c_bj[k];
+    const int n = c_n[k];
     float out = __int_as_float(0x7fc00000);
     if (bj >= 0) {
-      const int k = S.cd[2 * x + bj];
-      const int d = dmin + k;
+      const int d = dmin + int((c_kk[k] >> (8 * bj)) & 0xFFu);
       const int xr = x - d;
       const uint32_t mine = 0xFFFFFFFFu - uint32_t(x);
       if (uint32_t(claim[xr] & 0xFFFFFFFFull) == mine) {
@@ -385,7 +371,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
           const uint32_t nb = recs[x].z;
           const float cm = nb_score(nb & 0xFFFFu), cp = nb_score(nb >> 16);
           if (cm > -1e29f && cp > -1e29f) {
-            const float c0 = S.cs[2 * x];
+            const float c0 = c_s[k][0];
             if (F.fit_eq) {
               const float lower = cm < cp ? cm : cp;
               const float den = c0 - lower;
@@ -406,15 +392,17 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
     float lrc2 = __int_as_float(0x7fc00000);
     if (fin) {
       const int xr = x - int(roundf(out));
-      if (xr >= 0 && xr < W && S.rev[xr] >= 0) lrc2 = fabsf(out - float(dmin + S.rev[xr]));
+      if (xr >= 0 && xr < W) {
+        const unsigned long long rk = S.revk[xr];
+        if (rk) lrc2 = fabsf(out - float(dmin + int(rk & 0xFFFFu)));
+      }
     }
-    const int n = S.cn[x];
     if (F.cand6) {
       DE_GLOBAL float* c6 = gptr(F.cand6) + row + x;
-      c6[0] = n > 0 ? S.cs[2 * x] : kNone;
-      c6[WH] = n > 1 ? S.cs[2 * x + 1] : kNone;
-      c6[2 * WH] = n > 0 ? float(dmin + S.cd[2 * x]) : kNone;
-      c6[3 * WH] = n > 1 ? float(dmin + S.cd[2 * x + 1]) : kNone;
+      c6[0] = c_s[k][0];
+      c6[WH] = c_s[k][1];
+      c6[2 * WH] = n > 0 ? float(dmin + int(c_kk[k] & 0xFFu)) : kNone;
+      c6[3 * WH] = n > 1 ? float(dmin + int(c_kk[k] >> 8)) : kNone;
       c6[4 * WH] = kNone;   // exact reverse match (--lrc) is written by k_lrc_exact
       c6[5 * WH] = lrc2;
     }
@@ -423,21 +411,18 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(BatchView b) {
     }
     if (F.conf) {
       const bool has = fin && out > 0.f;
-      gptr(F.conf)[row + x] = has ? uint8_t(roundf(confidence_dev(n > 0 ? S.cs[2 * x] : kNone,
-                                                            n > 1 ? S.cs[2 * x + 1] : kNone,
-                                                            lambda, (lrc2 == lrc2) ? lrc2 : 0.f,
-                                                            1.f) * 255.f))
-                            : uint8_t(0);
+      gptr(F.conf)[row + x] =
+          has ? uint8_t(roundf(confidence_dev(c_s[k][0], c_s[k][1], lambda,
+                                              (lrc2 == lrc2) ? lrc2 : 0.f, 1.f) * 255.f))
+              : uint8_t(0);
     }
-    (void)q;
   }
 }
 
 size_t solve_lds_bytes(int W) {
   auto r = [](size_t v) { return (v + 15) & ~size_t(15); };
   return r(8 * size_t(W)) * 3 + r(4 * size_t(W + 1)) + r(4 * size_t(W)) * 2 +
-         r(4 * size_t(kSolveThreads)) + r(4 * size_t(W)) * 2 + r(2 * size_t(W)) +
-         r(size_t(W)) * 2;
+         r(4 * size_t(kSolveThreads));
 }
 
 // maxW: widest row of the launch (at most kMaxW).
