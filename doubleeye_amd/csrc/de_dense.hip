@@ -64,6 +64,26 @@ __device__ __forceinline__ int local_item(const BatchView& b, int stage, int ite
   return item - b.start[stage * (b.n + 1) + f];
 }
 
+// Census of the pixel at (cy, cx) of an LDS tile: bit b is set when window
+// position b (row-major over (2HW+1) x (2HH+1), the centre skipped) is darker
+// than the centre. The bits enter from the last position down: n - c (bytes,
+// so bit 31 is n < c) is funnel-shifted into the word, acc = (acc << 1) |
+// ((n - c) >> 31), two instructions a bit and no lane masks.
+template <int HW, int HH, int TW>
+__device__ __forceinline__ uint64_t census_at(const uint8_t (*t)[TW], int cy, int cx) {
+  constexpr int NWX = 2 * HW + 1, NB = NWX * (2 * HH + 1) - 1, C = NB / 2;
+  const uint32_t c = t[cy][cx];
+  uint32_t hi = 0, lo = 0;
+#pragma unroll
+  for (int b = NB - 1; b >= 0; --b) {
+    const int p = b < C ? b : b + 1;
+    const uint32_t diff = uint32_t(t[cy + p / NWX - HH][cx + p % NWX - HW]) - c;
+    if (b >= 32) hi = __builtin_amdgcn_alignbit(hi, diff, 31);
+    else lo = __builtin_amdgcn_alignbit(lo, diff, 31);
+  }
+  return uint64_t(lo) | (uint64_t(hi) << 32);
+}
+
 // ---------------------------------------------------------------------------
 // Stage 1: census of both images, left record, transposed vertical coefficients.
 // Tile of 64 x 8 pixels staged in LDS with its census halo; 256 threads, two
@@ -103,17 +123,8 @@ __global__ __launch_bounds__(256) void k_prep(BatchView b) {
     const int cy = r + HH, cx = lx + HW;
     uint64_t vl = 0, vr = 0;
     if (x >= HW && x < W - HW && y >= HH && y < H - HH) {
-      const uint8_t ml = sl[cy][cx], mr = sr[cy][cx];
-      int bit = 0;
-#pragma unroll
-      for (int dy = -HH; dy <= HH; ++dy)
-#pragma unroll
-        for (int dx = -HW; dx <= HW; ++dx) {
-          if (dx == 0 && dy == 0) continue;
-          vl |= uint64_t(sl[cy + dy][cx + dx] < ml) << bit;
-          vr |= uint64_t(sr[cy + dy][cx + dx] < mr) << bit;
-          ++bit;
-        }
+      vl = census_at<HW, HH, TW>(sl, cy, cx);
+      vr = census_at<HW, HH, TW>(sr, cy, cx);
     }
     const int lc = sl[cy][cx];
     const uint32_t ax = x ? srf[abs(lc - int(sl[cy][cx - 1]))] : srf[0];
@@ -779,17 +790,7 @@ __global__ __launch_bounds__(256) void k_census_plane(const uint8_t* __restrict_
     if (x >= W || y >= H) continue;
     uint64_t v = 0;
     if (x >= HW && x < W - HW && y >= HH && y < H - HH) {
-      const int cy = r + HH, cx = lx + HW;
-      const uint8_t m = s[cy][cx];
-      int bit = 0;
-#pragma unroll
-      for (int dy = -HH; dy <= HH; ++dy)
-#pragma unroll
-        for (int dx = -HW; dx <= HW; ++dx) {
-          if (dx == 0 && dy == 0) continue;
-          v |= uint64_t(s[cy + dy][cx + dx] < m) << bit;
-          ++bit;
-        }
+      v = census_at<HW, HH, TW>(s, r + HH, lx + HW);
     }
     out[size_t(y) * W + x] = v;
   }
