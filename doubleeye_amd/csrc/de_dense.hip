@@ -1423,56 +1423,91 @@ __global__ __launch_bounds__(64) void k_mcol(const FrameDesc* __restrict__ fp,
   }
 }
 
-// Plane k's score at (x, y) is a candidate: inside the rectangle, the plane's
-// row range and the row's interval.
-__device__ __forceinline__ bool mask_inserts(const int4& r, const int4& b, const int* mx0,
-                                             const int* mx1, int D, int y, int x, int k) {
-  if (r.y <= r.x) return false;
-  if (y < max(r.x, b.x) || y >= min(r.y, b.y + 1)) return false;
-  const int xa = max(r.z, mx0[size_t(y) * D + k]);
-  const int xb = min(r.w, mx1[size_t(y) * D + k] + 1);
-  return x >= xa && x < xb;
+// Plane k's candidates in row y: columns [xa, xb), the rectangle's x range
+// cut to the row's interval; empty (xb <= xa) outside the rectangle's and the
+// plane's row ranges.
+__device__ __forceinline__ int2 mask_span(const int4* rect, const int4* box, const int* mx0,
+                                          const int* mx1, int D, int y, int k) {
+  const int4 r = rect[k], b = box[k];
+  if (r.y <= r.x || y < max(r.x, b.x) || y >= min(r.y, b.y + 1)) return make_int2(0, 0);
+  return make_int2(max(r.z, mx0[size_t(y) * D + k]), min(r.w, mx1[size_t(y) * D + k] + 1));
+}
+
+// The spans of every plane in row blockIdx.y, one thread a plane, into LDS
+// (D <= kMaxD = the workgroup's 256 threads; planes from D on are empty).
+__device__ __forceinline__ void row_spans(int2* span, const FrameDesc& F, const int4* rect,
+                                          const int4* box, const int* mx0, const int* mx1) {
+  const int t = threadIdx.x;
+  span[t] = t < F.D ? mask_span(rect, box, mx0, mx1, F.D, blockIdx.y, t) : make_int2(0, 0);
+  __syncthreads();
 }
 
 // Top-2 per pixel over the inserted planes (ascending k, strictly greater
-// displaces: the packed keys), no sub-pixel neighbours.
+// displaces: the packed keys, so a key of 0 changes nothing), no sub-pixel
+// neighbours. The grid's rows are the image rows. A pixel's planes go 8 at a
+// time: one 16-byte volume load for a group holding an inserted plane, none
+// for the others (lanes are pixels 2*Dpad bytes apart, so every load touches
+// a line per lane; loading a group once instead of each plane alone cuts
+// the lines touched).
 __global__ __launch_bounds__(256) void k_minsert(const FrameDesc* __restrict__ fp,
                                                  const int4* __restrict__ rect,
                                                  const int4* __restrict__ box,
                                                  const int* __restrict__ mx0,
                                                  const int* __restrict__ mx1) {
+  static_assert(kMaxD == 256, "one thread per plane");
+  __shared__ int2 span[kMaxD];
   const FrameDesc F = *fp;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= F.W * F.H) return;
-  const int y = i / F.W, x = i - y * F.W;
+  row_spans(span, F, rect, box, mx0, mx1);
+  const int y = blockIdx.y;
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  if (x >= F.W) return;
+  const size_t i = size_t(y) * F.W + x;
+  const uint4* pv = reinterpret_cast<const uint4*>(F.vol + i * F.Dpad);
   uint32_t p0 = 0u, p1 = 0u;
-  for (int k = 0; k < F.D; ++k) {
-    if (!mask_inserts(rect[k], box[k], mx0, mx1, F.D, y, x, k)) continue;
-    const uint32_t key = pack_key(F.vol[size_t(i) * F.Dpad + k], k);
-    p1 = med3(p0, p1, key);
-    p0 = max(p0, key);
+  for (int k0 = 0; k0 < F.D; k0 += 8) {                  // k0 + 7 < Dpad
+    uint32_t m = 0u;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int2 sp = span[k0 + u];
+      m |= uint32_t(x >= sp.x && x < sp.y) << u;
+    }
+    if (m) {
+      const uint4 v = pv[k0 >> 3];
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t vv = (u & 1) ? (w[u >> 1] >> 16) : (w[u >> 1] & 0xFFFFu);
+        const uint32_t key = (m >> u & 1u) ? pack_key(vv, k0 + u) : 0u;
+        p1 = med3(p0, p1, key);
+        p0 = max(p0, key);
+      }
+    }
   }
   reinterpret_cast<uint4*>(F.cand)[i] = make_uint4(p0, p1, 0xFFFFFFFFu, 0u);
 }
 
-// Exact reverse match over the inserted scores only.
+// Exact reverse match over the inserted scores only (grid rows = image rows,
+// as k_minsert). Right pixel xr meets plane k at left pixel xr + dmin + k.
 __global__ __launch_bounds__(256) void k_mlrc(const FrameDesc* __restrict__ fp,
                                               const int4* __restrict__ rect,
                                               const int4* __restrict__ box,
                                               const int* __restrict__ mx0,
                                               const int* __restrict__ mx1) {
+  __shared__ int2 span[kMaxD];
   const FrameDesc F = *fp;
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= F.W * F.H) return;
-  const int y = j / F.W, xr = j - y * F.W;
+  row_spans(span, F, rect, box, mx0, mx1);
+  const int y = blockIdx.y;
+  const int xr = blockIdx.x * 256 + threadIdx.x;
+  if (xr >= F.W) return;
   uint32_t key = 0u;
   for (int k = 0; k < F.D; ++k) {
     const int x = xr + F.dmin + k;
     if (x >= F.W) break;
-    if (!mask_inserts(rect[k], box[k], mx0, mx1, F.D, y, x, k)) continue;
-    key = max(key, pack_key(F.vol[(size_t(y) * F.W + x) * F.Dpad + k], k));
+    const int2 sp = span[k];
+    if (x >= sp.x && x < sp.y)
+      key = max(key, pack_key(F.vol[(size_t(y) * F.W + x) * F.Dpad + k], k));
   }
-  F.rdisp[j] = key ? int16_t(key_k(key)) : int16_t(-1);
+  F.rdisp[size_t(y) * F.W + xr] = key ? int16_t(key_k(key)) : int16_t(-1);
 }
 
 hipError_t launch_masked(const FrameDesc& hf, const FrameDesc* df, const float* prior, int band,
@@ -1501,11 +1536,9 @@ hipError_t launch_masked(const FrameDesc& hf, const FrameDesc* df, const float* 
     default:
       return hipErrorInvalidValue;
   }
-  hipLaunchKernelGGL(k_minsert, dim3((WH + 255) / 256), dim3(256), 0, s, df, m.rect, m.box,
-                     m.mx0, m.mx1);
-  if (lrc)
-    hipLaunchKernelGGL(k_mlrc, dim3((WH + 255) / 256), dim3(256), 0, s, df, m.rect, m.box,
-                       m.mx0, m.mx1);
+  const dim3 rows((hf.W + 255) / 256, hf.H);
+  hipLaunchKernelGGL(k_minsert, rows, dim3(256), 0, s, df, m.rect, m.box, m.mx0, m.mx1);
+  if (lrc) hipLaunchKernelGGL(k_mlrc, rows, dim3(256), 0, s, df, m.rect, m.box, m.mx0, m.mx1);
   return hipGetLastError();
 }
 
