@@ -2,10 +2,11 @@
 iterations (the C2 workload), through the C ABI with inputs resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--frames-in-flight N]
 
 Prints ONE JSON line. `value` = Mpx*disp/s of the whole job (W*H*D per frame
 over the timed region). Frames run as a video stream would: two frames in
-flight, each a captured plan (its own scratch and hipGraph) on its own HIP
+flight (--frames-in-flight), each a captured plan (its own scratch and hipGraph) on its own HIP
 stream, launched alternately, so one frame's row pass overlaps the other's
 column pass (the reference GPU tool pipelines two frames as well,
 core/tools/de_dense_cuda.cu:788-841); every frame runs all four stages and
@@ -35,7 +36,6 @@ W, H, D = 848, 480, 128
 CFG = dict(dmax=128, iters=8, sigma_s=20.0, sigma_r=0.0392)
 REF_FLAGS = ["--dmax", "128", "--iters", "8", "--sigma-s", "20", "--sigma-r", "0.0392"]
 HBM_PEAK_GBS = 8000.0
-FRAMES_IN_FLIGHT = 2
 
 
 def algorithmic_bytes(w: int, h: int, d: int) -> dict:
@@ -741,6 +741,8 @@ def main():
                          "batch; c3: keypoint-graph BP; c5: serving loop; sparse: keypoint "
                          "match_masda; detect: keypoint detector; pipe: de_pipe's pair")
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
+    ap.add_argument("--frames-in-flight", type=int, default=2, choices=[1, 2, 3, 4],
+                    help="c2: frames in flight, each on its own HIP stream (1: one at a time)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one plain device-to-device copy of MIB MiB (the "
                          "known-size kernel that scales rocprofv3 FETCH/WRITE_SIZE)")
@@ -783,8 +785,10 @@ def main():
     # Device memory and streams come from the library's own HIP runtime; no
     # second runtime (e.g. torch's bundled one) is loaded into the process.
     L, R, _ = synth.slanted_pair(W, H, seed=2)
+    nfl = a.frames_in_flight
     ctx = Context(0)
-    ctx2 = Context(0)                   # a second HIP stream for the second frame in flight
+    # one more context per further frame in flight, for its HIP stream
+    extra = [Context(0) for _ in range(nfl - 1)]
     if a.calibrate_copy:
         n = a.calibrate_copy << 20
         src = ctx.to_device(np.ones(n, np.uint8))
@@ -805,20 +809,21 @@ def main():
     ctx.sync()
     first = disp.numpy()
 
-    # two frames in flight: a plan per frame slot, slot k on stream k
+    # frames in flight: a plan per frame slot, slot k on stream k
     slots = []
-    for _ in range(FRAMES_IN_FLIGHT):
+    for _ in range(nfl):
         o = (ctx.empty((H, W), np.float32), ctx.empty((H, W), np.float32))
         slots.append((ctx.plan([Pair(dL.ptr, dR.ptr, W, H, o[0].ptr, o[1].ptr)], [cfg]), o))
-    streams = [ctx.stream, ctx2.stream]
+    streams = [ctx.stream] + [c.stream for c in extra]
 
     def run_inflight(k_steps):
         for k in range(k_steps):
-            slots[k % FRAMES_IN_FLIGHT][0].launch(streams[k % FRAMES_IN_FLIGHT])
+            slots[k % nfl][0].launch(streams[k % nfl])
 
     def sync_all():
         ctx.sync()
-        ctx2.sync()
+        for c in extra:
+            c.sync()
 
     run_inflight(a.warmup)
     sync_all()
@@ -858,7 +863,7 @@ def main():
     # HBM bytes per launch of each kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE
     # on this workload (profiles/dense_c2_848x480x128_kernels.md), scaled by the
     # known-size copy/fill calibration (reads x2.00, writes x1.00)
-    traffic = {"row": 312.6e6, "col": 308.0e6, "solve": 10.0e6, "prep": 15.5e6}
+    traffic = {"row": 312.6e6, "col": 308.4e6, "solve": 10.0e6, "prep": 15.5e6}
     achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
     frame_gbs = ab["frame"] / (ms * 1e-3) / 1e9
     out = {
@@ -877,7 +882,7 @@ def main():
         "config": {"workload": "dense MASDA 848x480, D=128 (d=1..128), census 7x7, "
                                "sigma_s 20, sigma_r 10/255, 8 BP iterations, top-2",
                    "W": W, "H": H, "D": D, "iters": 8,
-                   "frames_in_flight": FRAMES_IN_FLIGHT},
+                   "frames_in_flight": nfl},
         "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
@@ -892,7 +897,8 @@ def main():
                                            a.cpu_budget)
     for p, _ in slots:
         p.close()
-    ctx2.close()
+    for c in extra:
+        c.close()
     ctx.close()
     print(json.dumps(out), flush=True)
 

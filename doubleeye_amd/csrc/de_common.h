@@ -42,7 +42,7 @@ constexpr int kOne = 1 << kQ;      // Q14 unit score
 constexpr int kOff = 1 << kQ;      // v' = v + kOff
 constexpr int kMaxD = 256;         // 8-bit disparity index in the packed key
 constexpr int kMaxG = 4;           // disparity groups of 64 (D <= 256)
-constexpr int kMaxW = 3200;        // row solver LDS budget: 48 bytes per pixel within 160 KB
+constexpr int kMaxW = 4096;        // row solver: 36 bytes of LDS per pixel, 8 pixels per thread
 constexpr float kNone = -1e30f;
 
 // Everything one frame needs on the device. All pointers are device memory.

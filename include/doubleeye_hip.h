@@ -94,7 +94,7 @@ int de_census_dev(de_ctx* ctx, const uint8_t* d_img, int W, int H, int hw, int h
  * default path (recursive filter, blockwise top-2, K = 2 row solve), equal to
  * `de_dense --threads 1`. Outputs: disp W*H float (NaN = no answer),
  * margin W*H float, cand6 (nullable) 6 planes s1, s2, d1, d2, lrc, lrc2 as
- * written by --out-conf (-1e30 = missing candidate). Rows up to 3200 pixels
+ * written by --out-conf (-1e30 = missing candidate). Rows up to 4096 pixels
  * (the row solver holds a row in LDS); wider images return DE_ERR_ARG. */
 int de_dense_run(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
                  const de_cfg* cfg, float* disp, float* margin, float* cand6);

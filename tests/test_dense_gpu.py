@@ -133,7 +133,7 @@ def test_9x7_extension_and_dmin0(ctx, oracle):
     ((33, 65), dict(dmax=3, min_margin=0.02, fit_eq=0)),
     ((30, 1100), dict(dmax=70, iters=3)),         # rows wider than the solver's threads
     ((24, 2500), dict(dmax=40, iters=3)),         # several pixels per solver thread
-    ((14, 3200), dict(dmax=24, iters=2)),         # the widest row supported
+    ((10, 4096), dict(dmax=24, iters=2)),         # the widest row supported
 ])
 def test_edge_shapes(ctx, oracle, shape, kw):
     rng = np.random.default_rng(sum(shape))
@@ -151,6 +151,6 @@ def test_invalid_arguments_raise(ctx):
             ctx.run_dense(L, L, dcfg(**kw))
     with pytest.raises(ValueError):
         ctx.run_dense(L, np.zeros((32, 31), np.uint8))
-    wide = np.zeros((8, 3201), np.uint8)          # one column beyond the row solver's LDS
+    wide = np.zeros((8, 4097), np.uint8)          # one column beyond the row solver
     with pytest.raises(DeError):
         ctx.run_dense(wide, wide, dcfg(dmax=16))
