@@ -20,8 +20,8 @@ extern const int kColVariants[kNumColVariants];
 constexpr int kVolPadRows = 64;
 constexpr int kAytPad = 64;       // words of padding around the transposed coefficients
 
-constexpr int kPrepTileW = 64;   // census/coefficient tile: 64 x 8 pixels
-constexpr int kPrepTileH = 8;
+constexpr int kPrepTileW = 64;   // census/coefficient tile: 64 x 4 pixels
+constexpr int kPrepTileH = 4;
 
 // Work items per frame for each stage.
 __host__ __device__ inline int stage_items(const FrameDesc& f, int stage) {
