@@ -86,8 +86,8 @@ __device__ __forceinline__ uint64_t census_at(const uint8_t (*t)[TW], int cy, in
 
 // ---------------------------------------------------------------------------
 // Stage 1: census of both images, left record, transposed vertical coefficients.
-// Tile of 64 x 8 pixels staged in LDS with its census halo; 256 threads, two
-// pixels each.
+// Tile of kPrepTileW x kPrepTileH pixels staged in LDS with its census halo;
+// 256 threads, a pixel row of the tile per wave and pass.
 template <int HW, int HH>
 __global__ __launch_bounds__(256) void k_prep(BatchView b) {
   constexpr int TW = kPrepTileW + 2 * HW;
