@@ -296,8 +296,12 @@ int Ctx::launch_stages(const Prepared& P, const BatchView& bv, hipStream_t s, in
   if ((stage_mask & 4) &&
       (e = per_g(kStageCol, [&](int i, int j, int base, int cnt) {
          bool store = false;
-         for (int q = i; q < j; ++q) store = store || fds[q].store_vol != 0;
-         return launch_col(bv, base, cnt, fds[i].G, store, P.colv, s);
+         int min_h = fds[i].H;
+         for (int q = i; q < j; ++q) {
+           store = store || fds[q].store_vol != 0;
+           min_h = std::min(min_h, fds[q].H);
+         }
+         return launch_col(bv, base, cnt, fds[i].G, store, P.colv, min_h, s);
        })) != hipSuccess)
     return hip_fail(e, "k_col");
   bool any_lrc = false;

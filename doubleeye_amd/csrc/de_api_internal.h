@@ -121,7 +121,7 @@ hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_
 hipError_t launch_row(const BatchView& b, int base, int items, int G, size_t lds, int variant,
                       hipStream_t s);
 hipError_t launch_col(const BatchView& b, int base, int items, int G, bool store, int variant,
-                      hipStream_t s);
+                      int min_h, hipStream_t s);
 hipError_t launch_solve(const BatchView& b, int items, size_t lds, int maxW, hipStream_t s);
 hipError_t launch_lrc_exact(const BatchView& b, int items, hipStream_t s);
 hipError_t launch_lrc_plane(const BatchView& b, int items, hipStream_t s);

@@ -583,7 +583,13 @@ __device__ __forceinline__ void top2_merge(uint32_t& p0, uint32_t& p1, uint32_t 
   p1 = n1;
 }
 
-template <int G, int U, bool STORE>
+// NRB: V-fwd blocks of 64 rows kept in registers. Rows yreg .. H-2 (yreg =
+// H-1-64*NRB, at least 1 for every frame of the launch) are filtered forward
+// into registers instead of the volume, and the first NRB V-bwd blocks read
+// them there: 2 * 64 * NRB bytes per lane that neither go to memory nor come
+// back. Row H-2-j sits in half j & 1 of register word j >> 1; the code over
+// those rows is unrolled, so every slot is a fixed register.
+template <int G, int U, bool STORE, int NRB>
 __global__ __launch_bounds__(64 * G) void k_col(BatchView b, int base) {
   constexpr int NWL = (G + 1) / 2;           // LDS words per lane per row
   constexpr int RS = kWave * NWL + 1;        // padded row stride (words)
@@ -591,6 +597,7 @@ __global__ __launch_bounds__(64 * G) void k_col(BatchView b, int base) {
   constexpr int S = G == 1 ? 1 : (G == 4 ? 4 : 2);   // lanes per row
   constexpr int NW = kWave * NWL;            // words per row
   constexpr int WPS = NW / S;                // words per lane
+  constexpr int NRR = kWave * NRB;           // rows kept in registers
   static_assert(RPW * S <= kWave && NW % S == 0, "transposed phase shape");
   static_assert(kWave % (2 * U) == 0 && 2 * U <= kVolPadRows, "prefetch depth");
   __shared__ uint32_t rb[kWave * RS];
@@ -611,16 +618,18 @@ __global__ __launch_bounds__(64 * G) void k_col(BatchView b, int base) {
   uint16_t* const rb16 = reinterpret_cast<uint16_t*>(rb);
   const int slot = 2 * ((g >> 1) * kWave + lane) + (g & 1);   // u16 index within a row
   uint32_t t = uint32_t(__builtin_amdgcn_raw_buffer_load_b16(rs(0), vo, 0, 0)) << 16;
+  const int yreg = H - 1 - NRR;                          // first register row
+  uint32_t rr[NRB > 0 ? NRR / 2 : 1];
 
-  // ---- V-fwd over rows 1..H-1, U rows in flight ----
-  {
-    auto ldf = [&](uint32_t (&v)[U], int y0) {           // rows y0 .. y0+U-1
-      const __amdgpu_buffer_rsrc_t rr = rs(y0);
+  // ---- V-fwd over rows 1 .. yend-1 in the volume, U rows in flight ----
+  auto ldf = [&](uint32_t (&v)[U], int y0) {             // rows y0 .. y0+U-1
+    const __amdgpu_buffer_rsrc_t rr0 = rs(y0);
 #pragma unroll
-      for (int u = 0; u < U; ++u) v[u] = __builtin_amdgcn_raw_buffer_load_b16(rr, vo, u * ysb, 0);
-    };
+    for (int u = 0; u < U; ++u) v[u] = __builtin_amdgcn_raw_buffer_load_b16(rr0, vo, u * ysb, 0);
+  };
+  {
     auto fwd = [&](const uint32_t (&v)[U], int y0, int n) {
-      const __amdgpu_buffer_rsrc_t rr = rs(y0);
+      const __amdgpu_buffer_rsrc_t rr0 = rs(y0);
       uint32_t ab[U];                                    // coefficients of the chunk, one
       __builtin_memcpy(ab, ay + y0, sizeof(ab));         // wide scalar load
 #pragma unroll
@@ -628,26 +637,50 @@ __global__ __launch_bounds__(64 * G) void k_col(BatchView b, int base) {
         if (u < n) {
           const uint32_t a2 = ab[u];
           t = rf_chain(a2, t, rf_acc(a2, v[u]));
-          __builtin_amdgcn_raw_buffer_store_b16(uint16_t(t >> 16), rr, vo, u * ysb, 0);
+          __builtin_amdgcn_raw_buffer_store_b16(uint16_t(t >> 16), rr0, vo, u * ysb, 0);
         }
       }
     };
+    const int yend = NRB > 0 ? yreg : H;
     uint32_t A[U], B[U];
     int y = 1;
     ldf(A, y);
-    for (; y + 2 * U <= H; y += 2 * U) {
+    for (; y + 2 * U <= yend; y += 2 * U) {
       ldf(B, y + U);
       fwd(A, y, U);
       ldf(A, y + 2 * U);
       fwd(B, y + U, U);
     }
-    if (y + U <= H) {
+    if (y + U <= yend) {
       ldf(B, y + U);
       fwd(A, y, U);
-      fwd(B, y + U, H - y - U);
+      fwd(B, y + U, yend - y - U);
     } else {
-      fwd(A, y, H - y);
+      fwd(A, y, yend - y);
     }
+  }
+  if constexpr (NRB > 0) {
+    // ---- V-fwd over the register rows yreg .. H-2, then row H-1 ----
+    uint32_t V[2][U];
+    ldf(V[0], yreg);
+#pragma unroll
+    for (int c = 0; c < NRR / U; ++c) {
+      const int y0 = yreg + c * U;
+      ldf(V[(c + 1) & 1], c + 1 < NRR / U ? y0 + U : H - 1);   // the last brings row H-1
+      uint32_t ab[U];
+      __builtin_memcpy(ab, ay + y0, sizeof(ab));
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t a2 = ab[u];
+        t = rf_chain(a2, t, rf_acc(a2, V[c & 1][u]));
+        const int j = NRR - 1 - (c * U + u);             // row y0 + u = H-2-j; odd j first
+        if (j & 1) rr[j >> 1] = t & 0xFFFF0000u;
+        else rr[j >> 1] |= t >> 16;
+      }
+    }
+    const uint32_t a2 = ay[H - 1];
+    t = rf_chain(a2, t, rf_acc(a2, V[(NRR / U) & 1][0]));
+    if (STORE) __builtin_amdgcn_raw_buffer_store_b16(uint16_t(t >> 16), rs(H - 1), vo, 0, 0);
   }
 
   // ---- V-bwd in blocks of 64 rows, transposed top-2 ----
@@ -657,52 +690,34 @@ __global__ __launch_bounds__(64 * G) void k_col(BatchView b, int base) {
   if (threadIdx.x == 0) rec[size_t(H - 1) * W] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
   // rows ytop, ytop-1, ..., ytop-U+1 (resource based at the lowest row index)
   auto ldb = [&](uint32_t (&v)[U], int ytop) {
-    const __amdgpu_buffer_rsrc_t rr = rs(ytop - U + 1);
+    const __amdgpu_buffer_rsrc_t rr0 = rs(ytop - U + 1);
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      v[u] = __builtin_amdgcn_raw_buffer_load_b16(rr, vo, (U - 1 - u) * ysb, 0);
+      v[u] = __builtin_amdgcn_raw_buffer_load_b16(rr0, vo, (U - 1 - u) * ysb, 0);
+  };
+  // one V-bwd row: value v of row ytop - u, coefficient a2 of the row above it
+  auto bwd1 = [&](uint32_t v, uint32_t a2, const __amdgpu_buffer_rsrc_t& rr0, int u, int ru) {
+    t = rf_chain(a2, t, rf_acc(a2, v));
+    const uint32_t vv = t >> 16;
+    if (STORE) __builtin_amdgcn_raw_buffer_store_b16(uint16_t(vv), rr0, vo, (U - 1 - u) * ysb, 0);
+    rb16[ru * (2 * RS) + slot] = uint16_t(vv);
   };
   auto bwd = [&](const uint32_t (&v)[U], int ytop, int u0, int n) {
-    const __amdgpu_buffer_rsrc_t rr = rs(ytop - U + 1);
+    const __amdgpu_buffer_rsrc_t rr0 = rs(ytop - U + 1);
     // coefficient of row ytop-u+1 = ab[U-1-u]: one wide scalar load per chunk
     uint32_t ab[U];
     __builtin_memcpy(ab, ay + (ytop - U + 2), sizeof(ab));
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (u < n) {
-        const uint32_t a2 = ab[U - 1 - u];
-        t = rf_chain(a2, t, rf_acc(a2, v[u]));
-        const uint32_t vv = t >> 16;
-        if (STORE) __builtin_amdgcn_raw_buffer_store_b16(uint16_t(vv), rr, vo, (U - 1 - u) * ysb, 0);
-        rb16[(u0 + u) * (2 * RS) + slot] = uint16_t(vv);
-      }
-    }
+    for (int u = 0; u < U; ++u)
+      if (u < n) bwd1(v[u], ab[U - 1 - u], rr0, u, u0 + u);
   };
   // transposed-phase ownership: lane = sp * RPW + ri owns share sp of row g*RPW + ri
   const int ri = int(lane) % RPW, sp = int(lane) / RPW;
   const int r = g * RPW + ri;
   const bool kfull = kmax >= NWL * 128 - 1;              // every word of the row valid
   const int nw = min(NW, (max(kmax, 0) >> 7) * kWave + kWave);
-  uint32_t A[U], B[U];
-  ldb(A, H - 2);
-  for (int yb = H - 2; yb >= 0; yb -= kWave) {          // block: rows yb, yb-1, ..., yb-63
-    const int nrow = min(kWave, yb + 1);
-    int u0 = 0;
-    for (; u0 + 2 * U <= nrow; u0 += 2 * U) {
-      ldb(B, yb - u0 - U);
-      bwd(A, yb - u0, u0, U);
-      ldb(A, yb - u0 - 2 * U);
-      bwd(B, yb - u0 - U, u0 + U, U);
-    }
-    if (u0 < nrow) {                                     // last block: fewer than 2U rows left
-      if (u0 + U <= nrow) {
-        ldb(B, yb - u0 - U);
-        bwd(A, yb - u0, u0, U);
-        bwd(B, yb - u0 - U, u0 + U, nrow - u0 - U);
-      } else {
-        bwd(A, yb - u0, u0, nrow - u0);
-      }
-    }
+  // top-2 of the block rows yb, yb-1, ..., yb-nrow+1 held in rb
+  auto top2_block = [&](int yb, int nrow) {
     __syncthreads();
     const int yy = yb - r;
     const bool own = sp < S && r < nrow;
@@ -765,6 +780,50 @@ __global__ __launch_bounds__(64 * G) void k_col(BatchView b, int base) {
       rec[size_t(yy) * W] = make_uint4(cand ? p0 : 0u, cand ? p1 : 0u, nbw, 0u);
     }
     __syncthreads();
+  };
+  if constexpr (NRB > 0) {
+    // the register blocks: block k holds rows H-2-64k .. H-65-64k, slots 64k ..
+#pragma unroll
+    for (int k = 0; k < NRB; ++k) {
+      const int yb = H - 2 - kWave * k;
+#pragma unroll
+      for (int q = 0; q < kWave / U; ++q) {
+        const int ytop = yb - q * U;
+        const __amdgpu_buffer_rsrc_t rr0 = rs(ytop - U + 1);
+        uint32_t ab[U];
+        __builtin_memcpy(ab, ay + (ytop - U + 2), sizeof(ab));
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int j = kWave * k + q * U + u;
+          const uint32_t v = (j & 1) ? (rr[j >> 1] >> 16) : (rr[j >> 1] & 0xFFFFu);
+          bwd1(v, ab[U - 1 - u], rr0, u, q * U + u);
+        }
+      }
+      top2_block(yb, kWave);
+    }
+  }
+  const int yb0 = H - 2 - NRR;                           // first block from the volume
+  uint32_t A[U], B[U];
+  ldb(A, yb0);
+  for (int yb = yb0; yb >= 0; yb -= kWave) {            // block: rows yb, yb-1, ..., yb-63
+    const int nrow = min(kWave, yb + 1);
+    int u0 = 0;
+    for (; u0 + 2 * U <= nrow; u0 += 2 * U) {
+      ldb(B, yb - u0 - U);
+      bwd(A, yb - u0, u0, U);
+      ldb(A, yb - u0 - 2 * U);
+      bwd(B, yb - u0 - U, u0 + U, U);
+    }
+    if (u0 < nrow) {                                     // last block: fewer than 2U rows left
+      if (u0 + U <= nrow) {
+        ldb(B, yb - u0 - U);
+        bwd(A, yb - u0, u0, U);
+        bwd(B, yb - u0 - U, u0 + U, nrow - u0 - U);
+      } else {
+        bwd(A, yb - u0, u0, nrow - u0);
+      }
+    }
+    top2_block(yb, nrow);
   }
 }
 
@@ -883,29 +942,41 @@ hipError_t launch_row(const BatchView& b, int base, int items, int G, size_t lds
   }
 }
 
-template <int G, bool STORE>
-static void launch_col_g(const BatchView& b, int base, int items, int variant, hipStream_t s) {
+template <int G, bool STORE, int NRB>
+static void launch_col_gn(const BatchView& b, int base, int items, int variant, hipStream_t s) {
   switch (variant) {
-    case 0: hipLaunchKernelGGL((k_col<G, 8, STORE>), dim3(items), dim3(64 * G), 0, s, b, base); break;
-    case 1: hipLaunchKernelGGL((k_col<G, 16, STORE>), dim3(items), dim3(64 * G), 0, s, b, base); break;
-    default: hipLaunchKernelGGL((k_col<G, 32, STORE>), dim3(items), dim3(64 * G), 0, s, b, base); break;
+    case 0: hipLaunchKernelGGL((k_col<G, 8, STORE, NRB>), dim3(items), dim3(64 * G), 0, s, b, base); break;
+    case 1: hipLaunchKernelGGL((k_col<G, 16, STORE, NRB>), dim3(items), dim3(64 * G), 0, s, b, base); break;
+    default: hipLaunchKernelGGL((k_col<G, 32, STORE, NRB>), dim3(items), dim3(64 * G), 0, s, b, base); break;
   }
+}
+
+// Two register blocks (128 rows, ~100 more VGPRs) when every frame has H >=
+// 130. Measured at 848x480x128: column pass 66 -> 62 us alone, frame 0.158
+// -> 0.152 ms with two frames in flight. Three blocks were faster alone (60
+// us) but slower in flight and batched, four (256 VGPRs) slower alone (76 us).
+template <int G, bool STORE>
+static void launch_col_g(const BatchView& b, int base, int items, int variant, int min_h,
+                         hipStream_t s) {
+  if (min_h >= 2 * kWave + 2) launch_col_gn<G, STORE, 2>(b, base, items, variant, s);
+  else launch_col_gn<G, STORE, 0>(b, base, items, variant, s);
 }
 
 // Column pass over the frames whose column items are [base, base + items);
 // every one of them must have exactly G disparity groups.
+// min_h: the smallest H of the launch's frames (sets the register blocks).
 hipError_t launch_col(const BatchView& b, int base, int items, int G, bool store, int variant,
-                      hipStream_t s) {
+                      int min_h, hipStream_t s) {
   if (variant < 0 || variant >= kNumColVariants) return hipErrorInvalidValue;
   switch (G * 2 + (store ? 1 : 0)) {
-    case 2: launch_col_g<1, false>(b, base, items, variant, s); break;
-    case 3: launch_col_g<1, true>(b, base, items, variant, s); break;
-    case 4: launch_col_g<2, false>(b, base, items, variant, s); break;
-    case 5: launch_col_g<2, true>(b, base, items, variant, s); break;
-    case 6: launch_col_g<3, false>(b, base, items, variant, s); break;
-    case 7: launch_col_g<3, true>(b, base, items, variant, s); break;
-    case 8: launch_col_g<4, false>(b, base, items, variant, s); break;
-    case 9: launch_col_g<4, true>(b, base, items, variant, s); break;
+    case 2: launch_col_g<1, false>(b, base, items, variant, min_h, s); break;
+    case 3: launch_col_g<1, true>(b, base, items, variant, min_h, s); break;
+    case 4: launch_col_g<2, false>(b, base, items, variant, min_h, s); break;
+    case 5: launch_col_g<2, true>(b, base, items, variant, min_h, s); break;
+    case 6: launch_col_g<3, false>(b, base, items, variant, min_h, s); break;
+    case 7: launch_col_g<3, true>(b, base, items, variant, min_h, s); break;
+    case 8: launch_col_g<4, false>(b, base, items, variant, min_h, s); break;
+    case 9: launch_col_g<4, true>(b, base, items, variant, min_h, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
