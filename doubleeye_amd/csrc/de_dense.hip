@@ -271,6 +271,26 @@ __device__ __forceinline__ uint32_t row_chain_fwd(const RowBlk& S, uint32_t t2,
   return t2;
 }
 
+// The same chain into a register block instead of the forward scratch.
+__device__ __forceinline__ uint32_t row_chain_reg(const RowBlk& S, uint32_t t2, uint4& out) {
+  uint32_t t[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    t2 = rf_chain_z(S.z[u], t2, __umul24(S.w[u], S.c[u]) + 32768u);
+    t[u] = t2;
+  }
+  out.x = __builtin_amdgcn_perm(t[1], t[0], 0x07060302u);
+  out.y = __builtin_amdgcn_perm(t[3], t[2], 0x07060302u);
+  out.z = __builtin_amdgcn_perm(t[5], t[4], 0x07060302u);
+  out.w = __builtin_amdgcn_perm(t[7], t[6], 0x07060302u);
+  return t2;
+}
+
+// Aligned forward blocks a row pass keeps in registers (64 VGPRs): the last
+// kRowRegBlocks of its pipelined blocks skip the forward scratch, and the
+// backward scan takes them first.
+constexpr int kRowRegBlocks = 16;
+
 // Row-pass context of one wave: volume row, forward scratch, transpose tile.
 struct RowOut {
   DE_GLOBAL uint8_t* vrow;   // volume, column 0 of this row and group (wave-uniform)
@@ -442,7 +462,14 @@ __global__ __launch_bounds__(256) void k_row(BatchView b, int base, int items, i
     *fwd_at(x) = uint16_t(t2 >> 16);
   }
   const int xm0 = min((xb + 7) & ~7, xend);              // aligned blocks [xm0, xm0 + 8 nblk)
-  const int nblk = max(0, ((xend & ~7) - xm0) >> 3);
+  const int nblk_all = max(0, ((xend & ~7) - xm0) >> 3);
+  // the last kRowRegBlocks aligned blocks [E - NB, E) go to registers when at
+  // least two blocks precede them
+  constexpr int NB = kRowRegBlocks;
+  const bool use_reg = nblk_all >= NB + 2;
+  const int nblk = use_reg ? nblk_all - NB : nblk_all;   // blocks through the scratch
+  const int E = (xm0 >> 3) + nblk_all;
+  uint4 fr[NB];
   int x = xb;
   for (; x < xm0; ++x) {
     const uint4 rec = lr[x];
@@ -488,6 +515,22 @@ __global__ __launch_bounds__(256) void k_row(BatchView b, int base, int items, i
     t2 = row_chain_fwd(S0, t2, o.frow + size_t(x >> 3) * 1024 + fo);
     x += 8;
   }
+  if (use_reg) {
+    // the same pipeline, unrolled, into fr[0 .. NB): S[j & 1] holds block j
+    // scored, R[(j + 1) & 1] block j + 1 loaded
+    RowRaw R[2];
+    RowBlk S[2];
+    row_load_raw(q, R[0], x, d);
+    row_load_raw(q, R[1], x + 8, d);
+    row_score_raw(q, R[0], S[0]);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j + 2 < NB) row_load_raw(q, R[j & 1], x + 8 * (j + 2), d);
+      if (j + 1 < NB) row_score_raw(q, R[(j + 1) & 1], S[(j + 1) & 1]);
+      t2 = row_chain_reg(S[j & 1], t2, fr[j]);
+    }
+    x += 8 * NB;
+  }
   for (; x < xend; ++x) {
     const uint4 rec = lr[x];
     t2 = rf_chain_z(rec.z, t2,
@@ -516,10 +559,24 @@ __global__ __launch_bounds__(256) void k_row(BatchView b, int base, int items, i
   if (((W - 1) & 7) == 0) row_flush<XB, true>(o, bhi);
   for (x = W - 2; x >= 8 * bhi; --x) back1(x);
   int bb = bhi - 1;                                      // next interior block
-  if (bb - P + 1 >= blo) {
-    // two register sets alternate between loading and filtering
-    uint4 A[P], B[P];
-    row_load_blocks<P>(A, o, bb);
+  if (use_reg)
+    for (; bb >= E; --bb) {                              // at most one block above E
+      uint4 v[1];
+      row_load_blocks<1>(v, o, bb);
+      t2 = row_back_blocks<1, XB>(v, t2, lr, bb, o);
+    }
+  if (use_reg) bb = E - NB - 1;                          // below the register blocks
+  // two register sets alternate between loading and filtering
+  uint4 A[P], B[P];
+  const bool grouped = bb - P + 1 >= blo;
+  if (grouped) row_load_blocks<P>(A, o, bb);             // in flight over the register blocks
+  if (use_reg) {
+    uint4 vr[NB];                                        // block E - 1 - k first
+#pragma unroll
+    for (int k = 0; k < NB; ++k) vr[k] = fr[NB - 1 - k];
+    t2 = row_back_blocks<NB, XB>(vr, t2, lr, E - 1, o);
+  }
+  if (grouped) {
     for (; bb - 3 * P + 1 >= blo; bb -= 2 * P) {
       row_load_blocks<P>(B, o, bb - P);
       t2 = row_back_blocks<P, XB>(A, t2, lr, bb, o);
