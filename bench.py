@@ -5,13 +5,13 @@ iterations (the C2 workload), through the C ABI with inputs resident in HBM.
                     [--frames-in-flight N]
 
 Prints ONE JSON line. `value` = Mpx*disp/s of the whole job (W*H*D per frame
-over the timed region). Frames run as a video stream would: two frames in
-flight (--frames-in-flight), each a captured plan (its own scratch and hipGraph) on its own HIP
-stream, launched alternately, so one frame's row pass overlaps the other's
-column pass (the reference GPU tool pipelines two frames as well,
-core/tools/de_dense_cuda.cu:788-841); every frame runs all four stages and
-yields the same output. `single_stream_ms_per_step` is the same run with one
-frame at a time. `roofline` is for the dominant kernel, timed live with
+over the timed region). Frames run as a video stream would, several in flight
+(--frames-in-flight, default 4: the hardware queues a process gets), each a
+captured plan (its own scratch and hipGraph) on its own HIP stream, launched
+in turn, so one frame's row pass overlaps another's column pass (the
+reference GPU tool pipelines two frames, core/tools/de_dense_cuda.cu:788-841);
+every frame runs all four stages and yields the same output.
+`single_stream_ms_per_step` is the same run with one frame at a time. `roofline` is for the dominant kernel, timed live with
 hipEvents recorded between the stage launches on the launching stream (one
 frame at a time, so the kernel runs alone);
 `cpu_baseline` times the reference's own CPU tool (oracle/_ref/de_dense,
@@ -741,8 +741,9 @@ def main():
                          "batch; c3: keypoint-graph BP; c5: serving loop; sparse: keypoint "
                          "match_masda; detect: keypoint detector; pipe: de_pipe's pair")
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
-    ap.add_argument("--frames-in-flight", type=int, default=2, choices=[1, 2, 3, 4],
-                    help="c2: frames in flight, each on its own HIP stream (1: one at a time)")
+    ap.add_argument("--frames-in-flight", type=int, default=4, choices=[1, 2, 3, 4],
+                    help="c2: frames in flight, each on its own HIP stream (1: one at a time; "
+                         "4 = the hardware queues a process gets by default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one plain device-to-device copy of MIB MiB (the "
                          "known-size kernel that scales rocprofv3 FETCH/WRITE_SIZE)")
