@@ -1008,19 +1008,25 @@ static void launch_col_gn(const BatchView& b, int base, int items, int variant, 
   }
 }
 
-// Two register blocks (128 rows, ~100 more VGPRs) when every frame has H >=
-// 130. Measured at 848x480x128: column pass 66 -> 62 us alone, frame 0.158
-// -> 0.152 ms with two frames in flight. Three blocks were faster alone (60
-// us) but slower in flight and batched, four (256 VGPRs) slower alone (76 us).
+// Register blocks by the frames' height and the launch's size. A launch of at
+// most two waves per SIMD keeps 3 blocks (192 rows, ~130 more VGPRs) when every
+// frame has H >= 194; a larger one 2 blocks (128 rows), which leave room for
+// more waves per SIMD; frames under 130 rows none. Measured at 848x480x128
+// (1696 waves): column pass 66 us without, 62 with 2 blocks, 60 with 3; four
+// frames in flight 0.141 / 0.137 ms a frame with 2 / 3 blocks. The 64-pair
+// batch ran 10.9 ms with 2 blocks, 11.9 with 3. Four blocks (256 VGPRs, AGPR
+// spills) took 76 us.
 template <int G, bool STORE>
 static void launch_col_g(const BatchView& b, int base, int items, int variant, int min_h,
                          hipStream_t s) {
-  if (min_h >= 2 * kWave + 2) launch_col_gn<G, STORE, 2>(b, base, items, variant, s);
+  int lds = 0, cus = 1;
+  device_shape(&lds, &cus);
+  const bool small = size_t(items) * G <= size_t(8) * cus;
+  if (small && min_h >= 3 * kWave + 2) launch_col_gn<G, STORE, 3>(b, base, items, variant, s);
+  else if (min_h >= 2 * kWave + 2) launch_col_gn<G, STORE, 2>(b, base, items, variant, s);
   else launch_col_gn<G, STORE, 0>(b, base, items, variant, s);
 }
 
-// Column pass over the frames whose column items are [base, base + items);
-// every one of them must have exactly G disparity groups.
 // min_h: the smallest H of the launch's frames (sets the register blocks).
 hipError_t launch_col(const BatchView& b, int base, int items, int G, bool store, int variant,
                       int min_h, hipStream_t s) {

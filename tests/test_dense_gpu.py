@@ -126,7 +126,8 @@ def test_9x7_extension_and_dmin0(ctx, oracle):
 @pytest.mark.parametrize("shape,kw", [
     ((240, 424), dict(dmax=192, iters=2)),        # G = 3, most pixels have few planes
     ((96, 130), dict(dmax=256, dmin=1)),          # D = 256, more than the width allows
-    ((140, 300), dict(dmax=250, iters=2)),        # G = 4 with the column pass's register rows
+    ((140, 300), dict(dmax=250, iters=2)),        # G = 4, two register blocks in the column pass
+    ((200, 260), dict(dmax=250, iters=2)),        # G = 4, three register blocks
     ((20, 300), dict(dmax=64, iters=4)),          # few rows
     ((7, 7), dict(dmax=4)),                       # one valid pixel at most
     ((6, 40), dict(dmax=8)),                      # no valid row
