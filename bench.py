@@ -681,7 +681,9 @@ def bench_c2f(a):
     8 iterations, through the full sweep, `--prior` (a full-resolution prior
     from the pair's ground truth, band 2) and `--c2f`, host images to host
     disparity. The CPU baseline is the reference `de_dense --c2f --threads 1`
-    (oracle/_ref/de_dense) on the same pair, one run."""
+    (oracle/_ref/de_dense) on the same pair, one run. `resident_ms` times the
+    full sweep and --c2f again with the pair and outputs resident in HBM (each
+    call followed by a stream wait)."""
     import numpy as np
 
     from doubleeye_amd import synth
@@ -703,6 +705,21 @@ def bench_c2f(a):
     ms_full, full = timed(lambda: ctx.run_dense(L, R, cfg))
     ms_prior, pri = timed(lambda: ctx.run_dense_prior(L, R, prior, cfg, 2))
     ms_c2f, c2f = timed(lambda: ctx.run_dense_c2f(L, R, cfg, 2))
+    # the same with the pair and the outputs resident in HBM (no PCIe copies)
+    H, W = L.shape
+    dL, dR = ctx.to_device(L), ctx.to_device(R)
+    dd, dm = ctx.empty((H, W), np.float32), ctx.empty((H, W), np.float32)
+
+    def resident(fn):
+        def run():
+            fn()
+            ctx.sync()
+        return timed(run)[0]
+
+    ms_full_dev = resident(lambda: ctx.run_dense_dev(dL.ptr, dR.ptr, W, H, cfg, dd.ptr, dm.ptr))
+    ms_c2f_dev = resident(lambda: ctx.run_dense_c2f_dev(dL.ptr, dR.ptr, W, H, cfg, dd.ptr,
+                                                        dm.ptr, 0, 2))
+    c2f_dev_same = bool(np.array_equal(dd.numpy().view(np.uint32), c2f[0].view(np.uint32)))
 
     def bad1(d):
         ok = np.isfinite(d) & np.isfinite(gt)
@@ -713,6 +730,8 @@ def bench_c2f(a):
         "value": round(ms_c2f, 3), "unit": "ms", "higher_is_better": False,
         "full_ms": round(ms_full, 3), "prior_ms": round(ms_prior, 3), "c2f_ms": round(ms_c2f, 3),
         "c2f_used_mask": bool(c2f[4]),
+        "resident_ms": {"full": round(ms_full_dev, 3), "c2f": round(ms_c2f_dev, 3),
+                        "c2f_equal_to_host_call": c2f_dev_same},
         "bad1_coverage": {"full": bad1(full[0]), "prior": bad1(pri[0]), "c2f": bad1(c2f[0])},
         "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": round(ms_c2f, 3),
         "scaling": "weak", "vs_baseline": None, "dtype": "int16 volume / f32 messages",

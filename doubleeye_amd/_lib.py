@@ -208,6 +208,7 @@ SIGNATURES = {
     "de_server_release": (I, [P, I, I]),
     "de_dense_run_prior": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, C.c_size_t, I, P, P, P]),
     "de_dense_run_c2f": (I, [P, P, P, I, I, C.POINTER(DeCfgC), I, P, P, P, P, C.POINTER(I)]),
+    "de_dense_run_c2f_dev": (I, [P, P, P, I, I, C.POINTER(DeCfgC), I, P, P, P, C.POINTER(I), P]),
     "de_detector_default_cfg": (None, [C.POINTER(DeDetectorCfgC)]),
     "de_detect_max_keypoints": (C.c_size_t, [I, I, C.POINTER(DeDetectorCfgC)]),
     "de_detect_keypoints_fast": (I, [P, P, I, I, C.POINTER(DeDetectorCfgC), P, I, C.POINTER(I),

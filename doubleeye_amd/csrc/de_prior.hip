@@ -297,4 +297,22 @@ int de_dense_run_c2f(de_ctx* c, const uint8_t* L, const uint8_t* R, int W, int H
   return fetch_outputs(x, W, H, dd, dm, dc, disp, margin, cand6);
 }
 
+int de_dense_run_c2f_dev(de_ctx* c, const uint8_t* d_L, const uint8_t* d_R, int W, int H,
+                         const de_cfg* cfg, int band, float* d_disp, float* d_margin,
+                         float* d_cand6, int* used_mask, void* stream) {
+  if (!c || !d_L || !d_R || !cfg || !d_disp || !d_margin)
+    return fail(DE_ERR_ARG, "de_dense_run_c2f_dev: null argument");
+  if (W < 2 || H < 2) return fail(DE_ERR_ARG, "c2f: image %dx%d has no half level", W, H);
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  Ctx& x = c->c;
+  DE_HIP(hipSetDevice(x.device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : x.stream;
+  int used = 0;
+  if (int rc = c2f_dev(x, d_L, d_R, W, H, *cfg, band, d_disp, d_margin, d_cand6, nullptr, &used,
+                       s))
+    return rc;
+  if (used_mask) *used_mask = used;
+  return DE_OK;
+}
+
 }  // extern "C"

@@ -243,6 +243,20 @@ class Context:
                                      _p(coarse), C.byref(used)))
         return disp, margin, cand, coarse, bool(used.value)
 
+    def run_dense_c2f_dev(self, dL: int, dR: int, W: int, H: int, cfg: Cfg, d_disp: int,
+                          d_margin: int, d_cand6: int = 0, band: int = 2,
+                          stream: int = 0) -> bool:
+        """de_dense_run_c2f_dev: --c2f on device addresses, queued on the
+        stream after one wait for the coarse level; True when the masked sweep
+        ran."""
+        used = C.c_int(0)
+        check(lib().de_dense_run_c2f_dev(self.h, C.c_void_p(dL), C.c_void_p(dR), W, H,
+                                         C.byref(cfg.to_c()), int(band), C.c_void_p(d_disp),
+                                         C.c_void_p(d_margin),
+                                         C.c_void_p(d_cand6) if d_cand6 else None,
+                                         C.byref(used), C.c_void_p(stream) if stream else None))
+        return bool(used.value)
+
     def volume(self, L, R, cfg: Cfg, k0: int, k1: int) -> np.ndarray:
         L = np.ascontiguousarray(L, np.uint8)
         R = np.ascontiguousarray(R, np.uint8)

@@ -300,6 +300,14 @@ int de_dense_run_c2f(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int
                      const de_cfg* cfg, int band, float* disp, float* margin, float* cand6,
                      float* coarse, int* used_mask);
 
+/* The same (core/tools/de_dense.cpp:1771-1966) on device pointers, queued on
+ * `stream` (null: the context's). The coarse level's hole count decides
+ * between the masked and the full sweep, so the call waits for the stream
+ * once, after the coarse level; it returns with the rest queued. */
+int de_dense_run_c2f_dev(de_ctx* ctx, const uint8_t* d_L, const uint8_t* d_R, int W, int H,
+                         const de_cfg* cfg, int band, float* d_disp, float* d_margin,
+                         float* d_cand6, int* used_mask, void* stream);
+
 /* ==== Keypoint detector and the detect -> match pipeline ===================
  * detect_keypoints_fast (core/src/preproc.cpp:436-545): FAST-9 candidates,
  * Shi-Tomasi response at candidates, greedy suppression within nms_radius in

@@ -115,3 +115,27 @@ def test_empty_mask_and_errors(ctx, oracle):
         ctx.run_dense_prior(L, R, bad, dcfg(dmax=32))
     with pytest.raises(DeError):
         ctx.run_dense_prior(L, R, prior[:10], dcfg(dmax=32))
+
+
+@pytest.mark.parametrize("shape,kw", [((480, 848), dict(dmax=127, iters=8, subpixel=0)),
+                                      ((121, 203), dict(dmax=40, iters=3, subpixel=0))])
+def test_c2f_device_pointers_equal_host_call(ctx, shape, kw):
+    """de_dense_run_c2f_dev on resident buffers gives the host call's outputs,
+    bit for bit, with and without the candidate planes."""
+    from doubleeye_amd import synth
+    H, W = shape
+    L, R, _ = synth.slanted_pair(W, H, seed=2, dhi=0.8 * kw["dmax"])[:3]
+    note(f"[c2f dev] {W}x{H} {kw}")
+    cfg = dcfg(**kw)
+    d, m, cand, _, used = ctx.run_dense_c2f(L, R, cfg, 2, want_cand=True)
+    dL, dR = ctx.to_device(L), ctx.to_device(R)
+    dd, dm = ctx.empty((H, W), np.float32), ctx.empty((H, W), np.float32)
+    dc = ctx.empty((6, H, W), np.float32)
+    used_dev = ctx.run_dense_c2f_dev(dL.ptr, dR.ptr, W, H, cfg, dd.ptr, dm.ptr, dc.ptr, 2)
+    ctx.sync()
+    assert used_dev == used
+    assert same_f32(dd.numpy(), d).all() and same_f32(dm.numpy(), m).all()
+    assert same_f32(dc.numpy(), cand).all()
+    used2 = ctx.run_dense_c2f_dev(dL.ptr, dR.ptr, W, H, cfg, dd.ptr, dm.ptr, 0, 2)
+    ctx.sync()
+    assert used2 == used and same_f32(dd.numpy(), d).all()
