@@ -676,6 +676,69 @@ def bench_pipe(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_full(a):
+    """The FP32 full-volume path (`de_dense --topk 0`: the float volume and the
+    full-volume row solver) on the C2 pair, 848x480, d 1..128, 8 iterations,
+    inputs resident in HBM, one frame at a time. The host-pointer call must give
+    the same disparities. CPU baseline: the reference `de_dense --topk 0
+    --threads 1` on the same pair, one run (its own census+cost+solve timer)."""
+    import numpy as np
+
+    from doubleeye_amd import synth
+    from doubleeye_amd.dense import Cfg, Context
+
+    L, R, _ = synth.slanted_pair(W, H, seed=2)
+    ctx = Context(0)
+    cfg = Cfg(**CFG)
+    dL, dR = ctx.to_device(L), ctx.to_device(R)
+    dd, dm = ctx.empty((H, W), np.float32), ctx.empty((H, W), np.float32)
+
+    def step():
+        ctx.run_dense_full_dev(dL.ptr, dR.ptr, W, H, cfg, dd.ptr, dm.ptr)
+
+    step()
+    ctx.sync()
+    first = dd.numpy()
+    reps = max(10, a.steps // 4)
+    for _ in range(max(2, a.warmup // 4)):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    host_d, _ = ctx.run_dense_full(L, R, cfg)
+    same = bool(np.array_equal(host_d.view(np.uint32), first.view(np.uint32)) and
+                np.array_equal(dd.numpy().view(np.uint32), first.view(np.uint32)))
+    out = {
+        "metric": "dense Mpx*disp/s, FP32 full-volume path (--topk 0) at 848x480x128",
+        "value": round(W * H * D / (ms * 1e-3) / 1e6, 1), "unit": "Mpx*disp/s",
+        "n_gpus": 1, "steps": reps, "warmup": max(2, a.warmup // 4), "ms_per_step": round(ms, 4),
+        "deterministic_and_equal_to_host_call": same,
+        "coverage": round(float(np.isfinite(first).mean()), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 volume / f32 messages",
+        "data": "synthetic (seeded slanted-plane pair, doubleeye_amd.synth.slanted_pair seed 2)",
+        "config": {"workload": "de_dense --topk 0, 848x480, d 1..128, 8 iterations"},
+    }
+    exe = os.path.join(ROOT, "oracle", "_ref", "de_dense")
+    if not a.no_cpu_baseline and os.access(exe, os.X_OK):
+        with tempfile.TemporaryDirectory() as td:
+            lp, rp = os.path.join(td, "l.y8"), os.path.join(td, "r.y8")
+            L.tofile(lp)
+            R.tofile(rp)
+            res = subprocess.run([exe, lp, rp, str(W), str(H), "--threads", "1", "--topk", "0",
+                                  *REF_FLAGS], capture_output=True, text=True, check=True).stdout
+            t = float(re.findall(r"total ([0-9.]+) ms", res)[-1])
+        out["cpu_baseline"] = {"value": round(W * H * D / (t * 1e-3) / 1e6, 2),
+                               "unit": "Mpx*disp/s", "cores": 1, "kind": "reference",
+                               "sample": f"one frame of de_dense --topk 0 --threads 1, {t:.0f} ms "
+                                         "(its own census+cost+solve timer)"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def bench_c2f(a):
     """The prior as a mask: one 848x480 slanted pair at D = 128 (d 0..127),
     8 iterations, through the full sweep, `--prior` (a full-resolution prior
@@ -769,7 +832,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--workload", default="c2",
                     choices=["c2", "c3", "c4", "c5", "sparse", "dense_masda", "detect",
-                             "pipe", "c2f"],
+                             "pipe", "c2f", "full"],
                     help="c2: one 848x480x128 pair (default, the headline); c4: 64-pair "
                          "batch; c3: keypoint-graph BP; c5: serving loop; sparse: keypoint "
                          "match_masda; detect: keypoint detector; pipe: de_pipe's pair")
@@ -809,6 +872,9 @@ def main():
         return
     if a.workload == "c2f":
         bench_c2f(a)
+        return
+    if a.workload == "full":
+        bench_full(a)
         return
 
     import numpy as np

@@ -195,8 +195,41 @@ __device__ __forceinline__ void window(int x, int W, int D, int dmin, int* lo, i
   *hi = b;
 }
 
+// The beta half-step's top-2 over a right pixel's claimants (x = xr + dmin + k)
+// from its packed keys (ordered value << 32 | ~k): the best key, and the best
+// key with another k. The same rules as wave_top2.
+struct Diag {
+  float b1, b2;
+  int i1;
+};
+__device__ __forceinline__ Diag diag_top2(uint64_t top, uint64_t sec) {
+  Diag t{kNone, kNone, -1};
+  if (top == 0) return t;
+  const float m = unordered(uint32_t(top >> 32));
+  if (!(m > kNone)) return t;
+  t.b1 = m;
+  t.i1 = int(0xFFFFFFFFu - uint32_t(top));
+  if (sec != 0) {
+    const float s2 = unordered(uint32_t(sec >> 32));
+    t.b2 = s2 > kNone ? s2 : kNone;
+  }
+  return t;
+}
+
+__device__ __forceinline__ uint64_t f32_key(float v, int k) {
+  return (uint64_t(ordered(v)) << 32) | uint64_t(0xFFFFFFFFu - uint32_t(k));
+}
+
+// Every pass walks the row's [x][k] planes with lanes over k (coalesced). The
+// beta half-step's top-2 along each right pixel's diagonal is built with LDS
+// 64-bit atomic maxima of packed keys: the best in the rho pass that writes
+// the rho values, the runner-up (best with another k) in a second pass. Keys
+// order by value then smaller k, as wave_top2's do, so the order of the
+// atomics does not matter. Two key sets alternate by iteration, so that the
+// beta update of iteration it - 1 runs at the head of iteration it's rho pass
+// (a pixel's own elements, before their top-2 over k).
 __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
-  extern __shared__ unsigned long long claim[];   // [W]
+  extern __shared__ unsigned long long claim[];   // [4W]: key sets; claim[W] at the end
   const int y = blockIdx.x;
   const int W = f.W, D = f.D, dmin = f.dmin;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -210,7 +243,8 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
     beta[t] = 0.f;
     rho[t] = 0.f;
   }
-  for (int t = threadIdx.x; t < W; t += kSolveT) claim[t] = 0ull;
+  auto dk = [&](int set, int which) { return claim + size_t(set * 2 + which) * W; };
+  for (int t = threadIdx.x; t < 2 * W; t += kSolveT) claim[t] = 0ull;   // set 0
   // margins (scores above -1e29 only)
   for (int x = wv; x < W; x += kSolveWaves) {
     int lo, hi;
@@ -230,8 +264,18 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
     }
   }
   __syncthreads();
+  // beta(x, k) from its diagonal's top-2 in key set `set` (the beta half-step)
+  auto beta_update = [&](int set, int x, int k, size_t i) {
+    const int xr = x - dmin - k;
+    if (xr < 0) return;
+    const Diag t = diag_top2(dk(set, 0)[xr], dk(set, 1)[xr]);
+    const float ex = (k == t.i1) ? t.b2 : t.b1;
+    beta[i] = keep * (vol[i] - fmax_ref(f.gamma, ex)) + f.damping * beta[i];
+  };
   for (int it = 0; it < f.iters; ++it) {
-    // rho: a left pixel's disparities compete
+    const int cs = it & 1, ps = cs ^ 1;
+    // beta of iteration it - 1, then rho: a left pixel's disparities compete;
+    // each new rho enters its diagonal's best key
     for (int x = wv; x < W; x += kSolveWaves) {
       int lo, hi;
       window(x, W, D, dmin, &lo, &hi);
@@ -241,7 +285,9 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
       for (int c = 0; c < nchunk; ++c) {
         const int k = c * kWave + lane;
         ok[c] = k >= lo && k < hi;
-        v[c] = ok[c] ? beta[size_t(x) * D + k] : 0.f;
+        const size_t i = size_t(x) * D + k;
+        if (ok[c] && it > 0) beta_update(ps, x, k, i);
+        v[c] = ok[c] ? beta[i] : 0.f;
       }
       const WTop2 t = wave_top2(v, ok, nchunk, lane);
       for (int c = 0; c < nchunk; ++c) {
@@ -250,40 +296,46 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
         const size_t i = size_t(x) * D + k;
         const float ex = (k == t.i1) ? t.b2 : t.b1;
         const float tgt = vol[i] - fmax_ref(f.lambda, ex);
-        rho[i] = keep * tgt + f.damping * rho[i];
+        const float r = keep * tgt + f.damping * rho[i];
+        rho[i] = r;
+        const int xr = x - dmin - k;
+        if (xr >= 0) atomicMax(&dk(cs, 0)[xr], f32_key(r, k));
       }
     }
     __syncthreads();
-    // beta: a right pixel's claimants compete (x = xr + dmin + k)
-    for (int xr = wv; xr < W; xr += kSolveWaves) {
-      const int kmax = min(D, W - xr - dmin);
-      if (kmax <= 0) continue;
-      float v[kMaxChunks], sv[kMaxChunks];
-      bool ok[kMaxChunks];
+    // the runner-up of each diagonal; the other key set is cleared for the
+    // next iteration (its last reads were in the pass above)
+    for (int t = threadIdx.x; t < 2 * W; t += kSolveT) dk(ps, 0)[t] = 0ull;
+    for (int x = wv; x < W; x += kSolveWaves) {
+      int lo, hi;
+      window(x, W, D, dmin, &lo, &hi);
       for (int c = 0; c < nchunk; ++c) {
         const int k = c * kWave + lane;
-        ok[c] = false;
-        if (k < kmax) {
-          const int x = xr + dmin + k;
-          int lo, hi;
-          window(x, W, D, dmin, &lo, &hi);
-          ok[c] = k >= lo && k < hi;
-        }
-        const size_t i = size_t(xr + dmin + k) * D + k;
-        v[c] = ok[c] ? rho[i] : 0.f;
-        sv[c] = ok[c] ? vol[i] : 0.f;
-      }
-      const WTop2 t = wave_top2(v, ok, nchunk, lane);
-      for (int c = 0; c < nchunk; ++c) {
-        if (!ok[c]) continue;
-        const int k = c * kWave + lane;
-        const size_t i = size_t(xr + dmin + k) * D + k;
-        const float ex = (k == t.i1) ? t.b2 : t.b1;
-        beta[i] = keep * (sv[c] - fmax_ref(f.gamma, ex)) + f.damping * beta[i];
+        const int xr = x - dmin - k;
+        if (k < lo || k >= hi || xr < 0) continue;
+        const uint64_t top = dk(cs, 0)[xr];
+        if (!(unordered(uint32_t(top >> 32)) > kNone)) continue;   // no best: no runner-up
+        if (k == int(0xFFFFFFFFu - uint32_t(top))) continue;
+        atomicMax(&dk(cs, 1)[xr], f32_key(rho[size_t(x) * D + k], k));
       }
     }
     __syncthreads();
   }
+  // the last iteration's beta half-step
+  if (f.iters > 0) {
+    const int ls = (f.iters - 1) & 1;
+    for (int x = wv; x < W; x += kSolveWaves) {
+      int lo, hi;
+      window(x, W, D, dmin, &lo, &hi);
+      for (int c = 0; c < nchunk; ++c) {
+        const int k = c * kWave + lane;
+        if (k >= lo && k < hi) beta_update(ls, x, k, size_t(x) * D + k);
+      }
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < W; t += kSolveT) claim[t] = 0ull;
+  __syncthreads();
   // best belief per left pixel, then the claim of its right pixel
   for (int x = wv; x < W; x += kSolveWaves) {
     int lo, hi;
@@ -409,15 +461,21 @@ static int f32_run(Ctx& x, const uint8_t* dL, const uint8_t* dR, int W, int H, c
   hipLaunchKernelGGL(k_f32_coeffs, dim3(unsigned((WH + 255) / 256)), dim3(256), 0, s, f);
   hipLaunchKernelGGL(k_f32_row, dim3(unsigned((H * G + 3) / 4)), dim3(256), 0, s, f);
   hipLaunchKernelGGL(k_f32_col, dim3(unsigned((W * G + 3) / 4)), dim3(256), 0, s, f);
-  if (solve)
-    hipLaunchKernelGGL(k_f32_solve, dim3(H), dim3(kSolveT), size_t(W) * 8, s, f);
+  if (solve) {
+    // 32 bytes of LDS per pixel of the row: above the default dynamic limit from W = 2049
+    static const hipError_t attr = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_f32_solve), hipFuncAttributeMaxDynamicSharedMemorySize,
+        160 * 1024);
+    DE_HIP(attr);
+    hipLaunchKernelGGL(k_f32_solve, dim3(H), dim3(kSolveT), size_t(W) * 32, s, f);
+  }
   DE_HIP(hipGetLastError());
   return DE_OK;
 }
 
 static int f32_validate(const de_cfg& c, int W, int H) {
   if (int rc = validate(c, W, H)) return rc;
-  if (size_t(W) * 8 > 160 * 1024) return fail(DE_ERR_ARG, "W=%d too wide for the row solver", W);
+  if (size_t(W) * 32 > 160 * 1024) return fail(DE_ERR_ARG, "W=%d too wide for the row solver", W);
   return DE_OK;
 }
 

@@ -287,6 +287,15 @@ class Context:
                                       _p(margin)))
         return disp, margin
 
+    def run_dense_full_dev(self, dL: int, dR: int, W: int, H: int, cfg: Cfg, d_disp: int,
+                           d_margin: int, stream: int = 0):
+        """de_dense_run_full_dev: `--topk 0` on device addresses, queued on the
+        stream (null: the context's)."""
+        check(lib().de_dense_run_full_dev(self.h, C.c_void_p(dL), C.c_void_p(dR), W, H,
+                                          C.byref(cfg.to_c()), C.c_void_p(d_disp),
+                                          C.c_void_p(d_margin),
+                                          C.c_void_p(stream) if stream else None))
+
     def candidates(self, L, R, cfg: Cfg) -> np.ndarray:
         L = np.ascontiguousarray(L, np.uint8)
         R = np.ascontiguousarray(R, np.uint8)
