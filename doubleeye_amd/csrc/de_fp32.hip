@@ -63,13 +63,6 @@ __device__ __forceinline__ uint32_t ordered(float v) {
 __device__ __forceinline__ float unordered(uint32_t u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
-__device__ __forceinline__ uint64_t wave_max64(uint64_t v) {
-  for (int o = 1; o < kWave; o <<= 1) {
-    const uint64_t w = __shfl_xor(v, o);
-    v = w > v ? w : v;
-  }
-  return v;
-}
 __device__ __forceinline__ float fmax_ref(float a, float b) { return (a < b) ? b : a; }
 
 // Sequential top-2 over values v[c] at k = c*64 + lane (valid[c]), in
@@ -80,37 +73,68 @@ struct WTop2 {
   float b1, b2;
   int i1;
 };
-__device__ WTop2 wave_top2(const float* v, const bool* valid, int nchunk, int lane) {
-  uint64_t key = 0;
+// Largest u32 over the wave (0: none) by DPP: row shifts 1, 2, 4, 8 leave each
+// row's maximum in its lane 15, two row broadcasts carry them to lane 63.
+// Every lane must be active.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x111, 0xF, 0xF, false)));
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x112, 0xF, 0xF, false)));
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x114, 0xF, 0xF, false)));
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x118, 0xF, 0xF, false)));
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x142, 0xA, 0xF, false)));
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x143, 0xC, 0xF, false)));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// The smallest k = c*64 + lane whose key is m (-1: none), by ballots in
+// ascending chunk order. Every lane must be active.
+__device__ __forceinline__ int wave_first(const uint32_t* key, int nchunk, uint32_t m) {
   for (int c = 0; c < nchunk; ++c) {
-    if (!valid[c]) continue;
-    const int k = c * kWave + lane;
-    const uint64_t kk = (uint64_t(ordered(v[c])) << 32) | uint64_t(0xFFFFFFFFu - uint32_t(k));
-    key = kk > key ? kk : key;
+    const uint64_t b = __ballot(key[c] == m);
+    if (b) return c * kWave + __ffsll(static_cast<unsigned long long>(b)) - 1;
   }
-  const uint64_t top = wave_max64(key);
+  return -1;
+}
+
+// The same order as packed keys (ordered value << 32 | ~k): the largest
+// value, the smallest k among equal values.
+__device__ WTop2 wave_top2(const float* v, const bool* valid, int nchunk, int lane) {
+  uint32_t key[kMaxChunks];
+  uint32_t lm = 0u;
+  for (int c = 0; c < nchunk; ++c) {
+    key[c] = valid[c] ? ordered(v[c]) : 0u;
+    lm = max(lm, key[c]);
+  }
   WTop2 t;
   t.b1 = kNone;
   t.b2 = kNone;
   t.i1 = -1;
-  if (top == 0) return t;
-  const float m = unordered(uint32_t(top >> 32));
-  if (!(m > kNone)) return t;
-  t.b1 = m;
-  t.i1 = int(0xFFFFFFFFu - uint32_t(top));
-  uint64_t key2 = 0;
-  for (int c = 0; c < nchunk; ++c) {
-    const int k = c * kWave + lane;
-    if (!valid[c] || k == t.i1) continue;
-    const uint64_t kk = (uint64_t(ordered(v[c])) << 32) | uint64_t(0xFFFFFFFFu - uint32_t(k));
-    key2 = kk > key2 ? kk : key2;
-  }
-  const uint64_t sec = wave_max64(key2);
-  if (sec != 0) {
-    const float s2 = unordered(uint32_t(sec >> 32));
+  const uint32_t m = wave_max_u32(lm);
+  if (m == 0u) return t;
+  const float b1 = unordered(m);
+  if (!(b1 > kNone)) return t;
+  t.b1 = b1;
+  t.i1 = wave_first(key, nchunk, m);
+  uint32_t lm2 = 0u;
+  for (int c = 0; c < nchunk; ++c)
+    if (c * kWave + lane != t.i1) lm2 = max(lm2, key[c]);
+  const uint32_t m2 = wave_max_u32(lm2);
+  if (m2 != 0u) {
+    const float s2 = unordered(m2);
     t.b2 = s2 > kNone ? s2 : kNone;
   }
   return t;
+}
+
+// A pixel's best belief key over its disparities: (ordered(bel) << 32 |
+// ~k) of the largest belief, smallest k among equals; 0 when none.
+__device__ __forceinline__ uint64_t wave_best_key(const uint32_t* key, int nchunk) {
+  uint32_t lm = 0u;
+  for (int c = 0; c < nchunk; ++c) lm = max(lm, key[c]);
+  const uint32_t m = wave_max_u32(lm);
+  if (m == 0u) return 0ull;
+  const int k = wave_first(key, nchunk, m);
+  return (uint64_t(m) << 32) | uint64_t(0xFFFFFFFFu - uint32_t(k));
 }
 
 __global__ void k_f32_coeffs(F32Desc f) {
@@ -340,19 +364,19 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
   for (int x = wv; x < W; x += kSolveWaves) {
     int lo, hi;
     window(x, W, D, dmin, &lo, &hi);
-    uint64_t key = 0;
+    uint32_t bk[kMaxChunks];
     for (int c = 0; c < nchunk; ++c) {
       const int k = c * kWave + lane;
+      bk[c] = 0u;
       if (k < lo || k >= hi) continue;
       const size_t i = size_t(x) * D + k;
       const float s = vol[i];
       if (s <= f.lambda) continue;
       const float bel = beta[i] + rho[i] - s;
       if (!(bel > -1e30f)) continue;
-      const uint64_t kk = (uint64_t(ordered(bel)) << 32) | uint64_t(0xFFFFFFFFu - uint32_t(k));
-      key = kk > key ? kk : key;
+      bk[c] = ordered(bel);
     }
-    const uint64_t top = wave_max64(key);
+    const uint64_t top = wave_best_key(bk, nchunk);
     if (lane == 0) {
       f.disp[size_t(y) * W + x] = NAN;
       if (top != 0) {
@@ -368,19 +392,19 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
   for (int x = wv; x < W; x += kSolveWaves) {
     int lo, hi;
     window(x, W, D, dmin, &lo, &hi);
-    uint64_t key = 0;
+    uint32_t bk[kMaxChunks];
     for (int c = 0; c < nchunk; ++c) {
       const int k = c * kWave + lane;
+      bk[c] = 0u;
       if (k < lo || k >= hi) continue;
       const size_t i = size_t(x) * D + k;
       const float s = vol[i];
       if (s <= f.lambda) continue;
       const float bel = beta[i] + rho[i] - s;
       if (!(bel > -1e30f)) continue;
-      const uint64_t kk = (uint64_t(ordered(bel)) << 32) | uint64_t(0xFFFFFFFFu - uint32_t(k));
-      key = kk > key ? kk : key;
+      bk[c] = ordered(bel);
     }
-    const uint64_t top = wave_max64(key);
+    const uint64_t top = wave_best_key(bk, nchunk);
     if (lane == 0 && top != 0) {
       const int k = int(0xFFFFFFFFu - uint32_t(top));
       const int d = dmin + k;
