@@ -252,6 +252,8 @@ __device__ __forceinline__ uint64_t f32_key(float v, int k) {
 // atomics does not matter. Two key sets alternate by iteration, so that the
 // beta update of iteration it - 1 runs at the head of iteration it's rho pass
 // (a pixel's own elements, before their top-2 over k).
+// NC: disparity chunks of 64 (ceil(D / 64)), a template so the chunk loops unroll.
+template <int NC>
 __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
   extern __shared__ unsigned long long claim[];   // [4W]: key sets; claim[W] at the end
   const int y = blockIdx.x;
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
   const float* vol = f.vol + rb;
   float* beta = f.beta + rb;
   float* rho = f.rho + rb;
-  const int nchunk = (D + kWave - 1) / kWave;
+  constexpr int nchunk = NC;
   const float keep = 1.f - f.damping;
   for (int t = threadIdx.x; t < W * D; t += kSolveT) {
     beta[t] = 0.f;
@@ -275,7 +277,8 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
     window(x, W, D, dmin, &lo, &hi);
     float v[kMaxChunks];
     bool ok[kMaxChunks];
-    for (int c = 0; c < nchunk; ++c) {
+    #pragma unroll
+      for (int c = 0; c < nchunk; ++c) {
       const int k = c * kWave + lane;
       ok[c] = k >= lo && k < hi;
       v[c] = ok[c] ? vol[size_t(x) * D + k] : 0.f;
@@ -306,6 +309,7 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
       if (hi <= lo) continue;
       float v[kMaxChunks];
       bool ok[kMaxChunks];
+      #pragma unroll
       for (int c = 0; c < nchunk; ++c) {
         const int k = c * kWave + lane;
         ok[c] = k >= lo && k < hi;
@@ -314,6 +318,7 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
         v[c] = ok[c] ? beta[i] : 0.f;
       }
       const WTop2 t = wave_top2(v, ok, nchunk, lane);
+      #pragma unroll
       for (int c = 0; c < nchunk; ++c) {
         if (!ok[c]) continue;
         const int k = c * kWave + lane;
@@ -333,6 +338,7 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
     for (int x = wv; x < W; x += kSolveWaves) {
       int lo, hi;
       window(x, W, D, dmin, &lo, &hi);
+      #pragma unroll
       for (int c = 0; c < nchunk; ++c) {
         const int k = c * kWave + lane;
         const int xr = x - dmin - k;
@@ -351,6 +357,7 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
     for (int x = wv; x < W; x += kSolveWaves) {
       int lo, hi;
       window(x, W, D, dmin, &lo, &hi);
+      #pragma unroll
       for (int c = 0; c < nchunk; ++c) {
         const int k = c * kWave + lane;
         if (k >= lo && k < hi) beta_update(ls, x, k, size_t(x) * D + k);
@@ -365,7 +372,8 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
     int lo, hi;
     window(x, W, D, dmin, &lo, &hi);
     uint32_t bk[kMaxChunks];
-    for (int c = 0; c < nchunk; ++c) {
+    #pragma unroll
+      for (int c = 0; c < nchunk; ++c) {
       const int k = c * kWave + lane;
       bk[c] = 0u;
       if (k < lo || k >= hi) continue;
@@ -393,7 +401,8 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
     int lo, hi;
     window(x, W, D, dmin, &lo, &hi);
     uint32_t bk[kMaxChunks];
-    for (int c = 0; c < nchunk; ++c) {
+    #pragma unroll
+      for (int c = 0; c < nchunk; ++c) {
       const int k = c * kWave + lane;
       bk[c] = 0u;
       if (k < lo || k >= hi) continue;
@@ -487,11 +496,21 @@ static int f32_run(Ctx& x, const uint8_t* dL, const uint8_t* dR, int W, int H, c
   hipLaunchKernelGGL(k_f32_col, dim3(unsigned((W * G + 3) / 4)), dim3(256), 0, s, f);
   if (solve) {
     // 32 bytes of LDS per pixel of the row: above the default dynamic limit from W = 2049
-    static const hipError_t attr = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_f32_solve), hipFuncAttributeMaxDynamicSharedMemorySize,
-        160 * 1024);
-    DE_HIP(attr);
-    hipLaunchKernelGGL(k_f32_solve, dim3(H), dim3(kSolveT), size_t(W) * 32, s, f);
+    const int nc = (D + kWave - 1) / kWave;
+    auto go = [&](auto kern) -> hipError_t {
+      const hipError_t a = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               160 * 1024);
+      if (a != hipSuccess) return a;
+      hipLaunchKernelGGL(kern, dim3(H), dim3(kSolveT), size_t(W) * 32, s, f);
+      return hipGetLastError();
+    };
+    hipError_t e;
+    if (nc == 1) e = go(k_f32_solve<1>);
+    else if (nc == 2) e = go(k_f32_solve<2>);
+    else if (nc == 3) e = go(k_f32_solve<3>);
+    else e = go(k_f32_solve<4>);
+    DE_HIP(e);
   }
   DE_HIP(hipGetLastError());
   return DE_OK;
