@@ -835,7 +835,8 @@ def main():
                              "pipe", "c2f", "full"],
                     help="c2: one 848x480x128 pair (default, the headline); c4: 64-pair "
                          "batch; c3: keypoint-graph BP; c5: serving loop; sparse: keypoint "
-                         "match_masda; detect: keypoint detector; pipe: de_pipe's pair")
+                         "match_masda; detect: keypoint detector; pipe: de_pipe's pair; "
+                         "c2f: the prior mask; full: the FP32 full-volume path (--topk 0)")
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
     ap.add_argument("--frames-in-flight", type=int, default=4, choices=[1, 2, 3, 4],
                     help="c2: frames in flight, each on its own HIP stream (1: one at a time; "
