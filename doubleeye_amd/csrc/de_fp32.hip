@@ -246,9 +246,10 @@ __device__ __forceinline__ uint64_t f32_key(float v, int k) {
 
 // Every pass walks the row's [x][k] planes with lanes over k (coalesced). The
 // beta half-step's top-2 along each right pixel's diagonal is built with LDS
-// 64-bit atomic maxima of packed keys: the best in the rho pass that writes
-// the rho values, the runner-up (best with another k) in a second pass. Keys
-// order by value then smaller k, as wave_top2's do, so the order of the
+// 64-bit atomic maxima of packed keys in the rho pass that writes the rho
+// values: the best, and the runner-up (the best with another k) from the
+// smaller of each key and the key it displaced. Keys are unique in a diagonal
+// and order by value then smaller k, as wave_top2's do, so the order of the
 // atomics does not matter. Two key sets alternate by iteration, so that the
 // beta update of iteration it - 1 runs at the head of iteration it's rho pass
 // (a pixel's own elements, before their top-2 over k).
@@ -328,27 +329,19 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
         const float r = keep * tgt + f.damping * rho[i];
         rho[i] = r;
         const int xr = x - dmin - k;
-        if (xr >= 0) atomicMax(&dk(cs, 0)[xr], f32_key(r, k));
+        if (xr >= 0) {
+          // best key; the smaller of this key and the one it displaced goes to
+          // the runner-up, so that every key but the final best ends there
+          const uint64_t key = f32_key(r, k);
+          const uint64_t old = atomicMax(&dk(cs, 0)[xr], key);
+          atomicMax(&dk(cs, 1)[xr], key > old ? old : key);
+        }
       }
     }
     __syncthreads();
-    // the runner-up of each diagonal; the other key set is cleared for the
-    // next iteration (its last reads were in the pass above)
+    // the other key set is cleared for the next iteration (its last reads
+    // were in the pass above)
     for (int t = threadIdx.x; t < 2 * W; t += kSolveT) dk(ps, 0)[t] = 0ull;
-    for (int x = wv; x < W; x += kSolveWaves) {
-      int lo, hi;
-      window(x, W, D, dmin, &lo, &hi);
-      #pragma unroll
-      for (int c = 0; c < nchunk; ++c) {
-        const int k = c * kWave + lane;
-        const int xr = x - dmin - k;
-        if (k < lo || k >= hi || xr < 0) continue;
-        const uint64_t top = dk(cs, 0)[xr];
-        if (!(unordered(uint32_t(top >> 32)) > kNone)) continue;   // no best: no runner-up
-        if (k == int(0xFFFFFFFFu - uint32_t(top))) continue;
-        atomicMax(&dk(cs, 1)[xr], f32_key(rho[size_t(x) * D + k], k));
-      }
-    }
     __syncthreads();
   }
   // the last iteration's beta half-step
