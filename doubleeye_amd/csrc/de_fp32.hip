@@ -146,6 +146,11 @@ __global__ void k_f32_coeffs(F32Desc f) {
   f.ay[i] = y ? f.rf[abs(l - int(f.L[i - f.W]))] : f.rf[0];
 }
 
+// The loops take their operands 8 steps at a time before the dependent
+// chain and its in-place stores (the compiler cannot move a load above a store
+// to the work plane, which it cannot prove disjoint), so 8 loads are in flight.
+constexpr int kF32Batch = 8;
+
 __global__ __launch_bounds__(256) void k_f32_row(F32Desc f) {
   const int item = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * 4 + int(threadIdx.x >> 6));
   if (item >= f.H * f.G) return;
@@ -155,21 +160,51 @@ __global__ __launch_bounds__(256) void k_f32_row(F32Desc f) {
   const int d = f.dmin + k;
   const bool live = k < f.D && y >= f.hh && y < f.H - f.hh;
   const size_t row = size_t(y) * f.W;
+  const int W = f.W;
   float fv = 0.f;
-  for (int x = 0; x < f.W; ++x) {
-    const uint64_t a = f.cl[row + x];
-    const float as = float(f.ax[row + x]) * kRfScale;
-    float P = 0.f;
-    if (live && x >= f.hw + d && x < f.W - f.hw)
-      P = (f.half - float(__popcll(a ^ f.cr[row + x - d]))) * f.inv;
-    fv = x == 0 ? P : P + as * (fv - P);
-    f.work[(row + x) * f.Dpad + k] = fv;
+  for (int x0 = 0; x0 < W; x0 += kF32Batch) {
+    float as[kF32Batch], P[kF32Batch];
+#pragma unroll
+    for (int u = 0; u < kF32Batch; ++u) {
+      const int x = x0 + u;
+      as[u] = 0.f;
+      P[u] = 0.f;
+      if (x < W) {
+        const uint64_t a = f.cl[row + x];
+        as[u] = float(f.ax[row + x]) * kRfScale;
+        if (live && x >= f.hw + d && x < W - f.hw)
+          P[u] = (f.half - float(__popcll(a ^ f.cr[row + x - d]))) * f.inv;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kF32Batch; ++u) {
+      const int x = x0 + u;
+      if (x < W) {
+        fv = x == 0 ? P[u] : P[u] + as[u] * (fv - P[u]);
+        f.work[(row + x) * f.Dpad + k] = fv;
+      }
+    }
   }
-  for (int x = f.W - 2; x >= 0; --x) {
-    const float F = f.work[(row + x) * f.Dpad + k];
-    const float as = float(f.ax[row + x + 1]) * kRfScale;
-    fv = F + as * (fv - F);
-    f.work[(row + x) * f.Dpad + k] = fv;
+  for (int x1 = W - 2; x1 >= 0; x1 -= kF32Batch) {       // columns x1, x1-1, ..., x1-7
+    float F[kF32Batch], as[kF32Batch];
+#pragma unroll
+    for (int u = 0; u < kF32Batch; ++u) {
+      const int x = x1 - u;
+      F[u] = 0.f;
+      as[u] = 0.f;
+      if (x >= 0) {
+        F[u] = f.work[(row + x) * f.Dpad + k];
+        as[u] = float(f.ax[row + x + 1]) * kRfScale;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kF32Batch; ++u) {
+      const int x = x1 - u;
+      if (x >= 0) {
+        fv = F[u] + as[u] * (fv - F[u]);
+        f.work[(row + x) * f.Dpad + k] = fv;
+      }
+    }
   }
 }
 
@@ -181,13 +216,29 @@ __global__ __launch_bounds__(256) void k_f32_col(F32Desc f) {
   const int k = g * kWave + lane;
   const int d = f.dmin + k;
   const size_t W = size_t(f.W);
+  const int H = f.H;
   float prev = f.work[size_t(x) * f.Dpad + k];
-  for (int y = 1; y < f.H; ++y) {
-    const size_t i = size_t(y) * W + x;
-    const float c = f.work[i * f.Dpad + k];
-    const float as = float(f.ay[i]) * kRfScale;
-    prev = c + as * (prev - c);
-    f.work[i * f.Dpad + k] = prev;
+  for (int y0 = 1; y0 < H; y0 += kF32Batch) {
+    float c[kF32Batch], as[kF32Batch];
+#pragma unroll
+    for (int u = 0; u < kF32Batch; ++u) {
+      const int y = y0 + u;
+      c[u] = 0.f;
+      as[u] = 0.f;
+      if (y < H) {
+        const size_t i = size_t(y) * W + x;
+        c[u] = f.work[i * f.Dpad + k];
+        as[u] = float(f.ay[i]) * kRfScale;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kF32Batch; ++u) {
+      const int y = y0 + u;
+      if (y < H) {
+        prev = c[u] + as[u] * (prev - c[u]);
+        f.work[(size_t(y) * W + x) * f.Dpad + k] = prev;
+      }
+    }
   }
   const bool xok = k < f.D && x >= f.hw + d && x < f.W - f.hw;
   auto emit = [&](int y, float v) {
@@ -195,13 +246,28 @@ __global__ __launch_bounds__(256) void k_f32_col(F32Desc f) {
       f.vol[(size_t(y) * W + x) * f.D + k] = (xok && y >= f.hh && y < f.H - f.hh) ? v : kNone;
   };
   float nxt = prev;
-  emit(f.H - 1, nxt);
-  for (int y = f.H - 2; y >= 0; --y) {
-    const size_t i = size_t(y) * W + x;
-    const float c = f.work[i * f.Dpad + k];
-    const float as = float(f.ay[i + W]) * kRfScale;
-    nxt = c + as * (nxt - c);
-    emit(y, nxt);
+  emit(H - 1, nxt);
+  for (int y1 = H - 2; y1 >= 0; y1 -= kF32Batch) {       // rows y1, y1-1, ..., y1-7
+    float c[kF32Batch], as[kF32Batch];
+#pragma unroll
+    for (int u = 0; u < kF32Batch; ++u) {
+      const int y = y1 - u;
+      c[u] = 0.f;
+      as[u] = 0.f;
+      if (y >= 0) {
+        const size_t i = size_t(y) * W + x;
+        c[u] = f.work[i * f.Dpad + k];
+        as[u] = float(f.ay[i + W]) * kRfScale;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kF32Batch; ++u) {
+      const int y = y1 - u;
+      if (y >= 0) {
+        nxt = c[u] + as[u] * (nxt - c[u]);
+        emit(y, nxt);
+      }
+    }
   }
 }
 
