@@ -1308,7 +1308,7 @@ __global__ __launch_bounds__(64) void k_mrow(const FrameDesc* __restrict__ fp,
                                              const int4* __restrict__ rect) {
   constexpr int Dp = G * kWave;
   constexpr uint32_t XB = 2u * Dp;                       // bytes per volume column
-  constexpr int P = 4;                                   // H-bwd blocks in flight
+  constexpr int P = 8;                                   // H-bwd blocks in flight
   extern __shared__ __align__(16) uint8_t smem[];
   const FrameDesc F = *fp;
   const int W = F.W;
