@@ -31,7 +31,7 @@ namespace de {
 
 namespace {
 
-constexpr int kSolveThreads = 512;   // 8 waves: two and more rows resident per CU
+constexpr int kSolveThreads = 512;   // 8 waves (16 for rows of 513..1024 pixels)
 
 __device__ __forceinline__ float fmax_ref(float a, float b) {   // std::max(a, b)
   return (a < b) ? b : a;
@@ -52,7 +52,7 @@ struct SolveSmem {
   int* bstart;      // [W + 1]
   int* cursor;      // [W]   placement cursors; item index of each bucket's best during the iterations
   int16_t* items;   // [2W]
-  int* scan;        // [kSolveThreads]
+  int* scan;        // [kSolveThreads] (wave totals: at most 16 used)
 };
 
 __device__ SolveSmem carve(uint8_t* base, int W) {
@@ -90,12 +90,12 @@ __device__ float confidence_dev(float s1, float s2, float lambda, float lrc, flo
 
 }  // namespace
 
-// KP: pixels (and buckets) per thread, at least ceil(W / kSolveThreads) for
-// every frame of the launch. Thread tid owns pixels x = tid + k*nt and the
+// KP: pixels (and buckets) per thread and NT threads, KP * NT at least the
+// widest row of the launch. Thread tid owns pixels x = tid + k*nt and the
 // buckets (right pixels) of the same indices, k < KP; everything about its
 // own pixels stays in its registers from the candidate parse to the outputs.
-template <int KP>
-__global__ __launch_bounds__(kSolveThreads) __attribute__((amdgpu_waves_per_eu(KP <= 4 ? 4 : 2)))
+template <int KP, int NT = kSolveThreads>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(KP <= 4 ? 4 : 2)))
 void k_solve(BatchView b) {
   extern __shared__ __align__(16) uint8_t smem_raw[];
   const int item = blockIdx.x;
@@ -427,9 +427,15 @@ size_t solve_lds_bytes(int W) {
 
 // maxW: widest row of the launch (at most kMaxW).
 hipError_t launch_solve(const BatchView& b, int items, size_t lds, int maxW, hipStream_t s) {
+  // Rows of 513..1024 pixels: 1024 threads of one pixel each (at 848x480x128
+  // 23.5 us against 26 with 512 threads of two; four frames in flight 0.136
+  // against 0.139 ms a frame, two in flight 0.157 against 0.153). Otherwise
+  // 512 threads of 1, 4 or 8 pixels.
   const int kp = (maxW + kSolveThreads - 1) / kSolveThreads;
   if (kp <= 1) hipLaunchKernelGGL(k_solve<1>, dim3(items), dim3(kSolveThreads), lds, s, b);
-  else if (kp == 2) hipLaunchKernelGGL(k_solve<2>, dim3(items), dim3(kSolveThreads), lds, s, b);
+  else if (kp == 2)
+    hipLaunchKernelGGL((k_solve<1, 2 * kSolveThreads>), dim3(items), dim3(2 * kSolveThreads), lds,
+                       s, b);
   else if (kp <= 4) hipLaunchKernelGGL(k_solve<4>, dim3(items), dim3(kSolveThreads), lds, s, b);
   else if (kp <= 8) hipLaunchKernelGGL(k_solve<8>, dim3(items), dim3(kSolveThreads), lds, s, b);
   else return hipErrorInvalidValue;
