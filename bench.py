@@ -964,7 +964,7 @@ def main():
     # HBM bytes per launch of each kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE
     # on this workload (profiles/dense_c2_848x480x128_kernels.md), scaled by the
     # known-size copy/fill calibration (reads x2.00, writes x1.00)
-    traffic = {"row": 282.0e6, "col": 245.9e6, "solve": 10.0e6, "prep": 18.3e6}
+    traffic = {"row": 282.0e6, "col": 245.9e6, "solve": 9.9e6, "prep": 18.3e6}
     achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
     frame_gbs = ab["frame"] / (ms * 1e-3) / 1e9
     out = {
