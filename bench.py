@@ -33,9 +33,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 W, H, D = 848, 480, 128
-CFG = dict(dmax=128, iters=8, sigma_s=20.0, sigma_r=0.0392)
+CFG7 = dict(dmax=128, iters=8, sigma_s=20.0, sigma_r=0.0392)
+CFG = dict(CFG7, census_hw=4, census_hh=3)                 # C2 as stated: census 9x7
 REF_FLAGS = ["--dmax", "128", "--iters", "8", "--sigma-s", "20", "--sigma-r", "0.0392"]
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of each stage kernel at C2 (census 9x7), from rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs on this workload scaled by a known-size copy and
+# fill; a quoted profile figure, not measured in this run (None: not profiled).
+TRAFFIC: dict = {}
+TRAFFIC_SOURCE = "not profiled for this build"
 
 
 def algorithmic_bytes(w: int, h: int, d: int) -> dict:
@@ -143,41 +149,52 @@ def bench_c4(a):
 
 
 def cpu_baseline(L, R, budget_s: float = 12.0):
-    """Reference CPU tool at --threads 1 on the same pair, repeated to ~budget_s, and
-    the same tool on up to 16 threads for half that time (reported beside it)."""
+    """Reference CPU tool (census 7x7, its only window) at --threads 1 on the
+    same pair, repeated to ~budget_s, and the same tool on up to 16 threads for
+    half that time (reported beside it). Falls back to the oracle port at 7x7
+    where the reference build is absent."""
     ref = os.path.join(ROOT, "oracle", "_ref", "de_dense")
+    if not os.access(ref, os.X_OK):
+        return cpu_baseline_port(L, R, budget_s, CFG7)
     with tempfile.TemporaryDirectory() as td:
         lp, rp = os.path.join(td, "l.y8"), os.path.join(td, "r.y8")
         L.tofile(lp)
         R.tofile(rp)
-        if os.access(ref, os.X_OK):
-            times = []
-            t_end = time.time() + budget_s
-            while time.time() < t_end or not times:
-                out = subprocess.run([ref, lp, rp, str(W), str(H), "--threads", "1", *REF_FLAGS],
-                                     capture_output=True, text=True, check=True).stdout
-                m = re.findall(r"total ([0-9.]+) ms", out)
-                times.append(float(m[-1]))
-            ms = sum(times) / len(times)
-            # the same tool on the host's share of cores (16 per GPU here), for context
-            nth = max(1, min(16, os.cpu_count() or 1))
-            mt = []
-            t_end = time.time() + budget_s / 2
-            while time.time() < t_end or not mt:
-                out = subprocess.run([ref, lp, rp, str(W), str(H), "--threads", str(nth),
-                                      *REF_FLAGS], capture_output=True, text=True,
-                                     check=True).stdout
-                mt.append(float(re.findall(r"total ([0-9.]+) ms", out)[-1]))
-            ms_mt = sum(mt) / len(mt)
-            return {"value": round(W * H * D / (ms * 1e-3) / 1e6, 2), "unit": "Mpx*disp/s",
-                    "cores": 1, "kind": "reference",
-                    "sample": f"{len(times)} frames of 848x480x128 iters 8, de_dense --threads 1 "
-                              f"(its own census+cost+solve timer), mean {ms:.1f} ms/frame",
-                    "all_threads": {"value": round(W * H * D / (ms_mt * 1e-3) / 1e6, 2),
-                                    "threads": nth, "ms_per_frame": round(ms_mt, 1),
-                                    "frames": len(mt)}}
+        times = []
+        t_end = time.time() + budget_s
+        while time.time() < t_end or not times:
+            out = subprocess.run([ref, lp, rp, str(W), str(H), "--threads", "1", *REF_FLAGS],
+                                 capture_output=True, text=True, check=True).stdout
+            m = re.findall(r"total ([0-9.]+) ms", out)
+            times.append(float(m[-1]))
+        ms = sum(times) / len(times)
+        # the same tool on the host's share of cores (16 per GPU here), for context
+        nth = max(1, min(16, os.cpu_count() or 1))
+        mt = []
+        t_end = time.time() + budget_s / 2
+        while time.time() < t_end or not mt:
+            out = subprocess.run([ref, lp, rp, str(W), str(H), "--threads", str(nth),
+                                  *REF_FLAGS], capture_output=True, text=True,
+                                 check=True).stdout
+            mt.append(float(re.findall(r"total ([0-9.]+) ms", out)[-1]))
+        ms_mt = sum(mt) / len(mt)
+    return {"value": round(W * H * D / (ms * 1e-3) / 1e6, 2), "unit": "Mpx*disp/s",
+            "cores": 1, "kind": "reference",
+            "sample": f"{len(times)} frames of 848x480x128 iters 8 census 7x7, de_dense "
+                      f"--threads 1 (its own census+cost+solve timer), mean {ms:.1f} ms/frame",
+            "all_threads": {"value": round(W * H * D / (ms_mt * 1e-3) / 1e6, 2),
+                            "threads": nth, "ms_per_frame": round(ms_mt, 1),
+                            "frames": len(mt)}}
+
+
+def cpu_baseline_port(L, R, budget_s: float = 12.0, cfg: dict | None = None):
+    """The oracle's restatement of the reference pipeline (single thread, the
+    reference's arithmetic) on the same pair, repeated to ~budget_s. The
+    reference's own tool has no 9x7 census, so the stated C2 workload is timed
+    through the port."""
     from oracle import binding as orc
-    c = orc.cfg(**CFG)
+    cfg = CFG if cfg is None else cfg
+    c = orc.cfg(**cfg)
     times = []
     t_end = time.time() + budget_s
     while time.time() < t_end or not times:
@@ -185,10 +202,11 @@ def cpu_baseline(L, R, budget_s: float = 12.0):
         orc.dense_run(L, R, c)
         times.append((time.perf_counter() - t0) * 1e3)
     ms = sum(times) / len(times)
+    win = f"{2 * cfg.get('census_hw', 3) + 1}x{2 * cfg.get('census_hh', 3) + 1}"
     return {"value": round(W * H * D / (ms * 1e-3) / 1e6, 2), "unit": "Mpx*disp/s",
             "cores": 1, "kind": "port",
-            "sample": f"{len(times)} frames of 848x480x128 iters 8 through the oracle, "
-                      f"mean {ms:.1f} ms/frame"}
+            "sample": f"{len(times)} frames of 848x480x128 iters 8 census {win} through the "
+                      f"oracle (C++ restatement of de_dense --threads 1), mean {ms:.1f} ms/frame"}
 
 
 def bench_sparse(a):
@@ -352,6 +370,10 @@ def bench_c3(a):
 
 
 def bench_c5(a):
+    print(json.dumps(run_c5(a.duration)), flush=True)
+
+
+def run_c5(duration: float) -> dict:
     """C5: serving loop. 16 camera streams, each 30 Hz with Poisson arrivals
     (exponential inter-arrival, mean 33.3 ms), alternately dense and
     keypoint-graph, sizes and disparity ranges drawn from the C4 sets with a
@@ -434,7 +456,7 @@ def bench_c5(a):
             last_note = time.perf_counter()
             print(f"[c5] {last_note - t0:.0f} s, {done} frames done", file=sys.stderr, flush=True)
         k = int(np.argmin(nxt))
-        if nxt[k] - t0 > a.duration + warm:
+        if nxt[k] - t0 > duration + warm:
             break
         while True:
             dt = nxt[k] - time.perf_counter()
@@ -478,12 +500,12 @@ def bench_c5(a):
         "dtype": "i16 volume / f32 messages",
         "data": "synthetic (slanted and fronto random-dot pairs, per-stream seeds)",
         "config": {"workload": f"16 streams (8 dense, 8 keypoint-graph), Poisson 30 Hz, "
-                               f"{a.duration:.0f} s, sizes/D from the C4 sets",
+                               f"{duration:.0f} s, sizes/D from the C4 sets",
                    "streams": [f"{st['kind']} {st['W']}x{st['H']} D{st['D']}" for st in streams]},
     }
     srv.close()
     ctx.close()
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_dense_masda(a):
@@ -841,6 +863,9 @@ def main():
     ap.add_argument("--frames-in-flight", type=int, default=4, choices=[1, 2, 3, 4],
                     help="c2: frames in flight, each on its own HIP stream (1: one at a time; "
                          "4 = the hardware queues a process gets by default)")
+    ap.add_argument("--c5-seconds", type=float, default=20.0,
+                    help="c2: seconds of the 16-stream serving leg reported beside the "
+                         "headline (p50/p99 latency); 0 skips it")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one plain device-to-device copy of MIB MiB (the "
                          "known-size kernel that scales rocprofv3 FETCH/WRITE_SIZE)")
@@ -881,7 +906,7 @@ def main():
     import numpy as np
 
     from doubleeye_amd import synth
-    from doubleeye_amd.dense import Cfg, Context, Pair
+    from doubleeye_amd.dense import Context
 
     # Device memory and streams come from the library's own HIP runtime; no
     # second runtime (e.g. torch's bundled one) is loaded into the process.
@@ -899,9 +924,83 @@ def main():
         ctx.sync()
         src.free(); dst.free()
     dL, dR = ctx.to_device(L), ctx.to_device(R)
+
+    m9 = measure_c2(ctx, extra, dL, dR, CFG, a)
+    m7 = measure_c2(ctx, extra, dL, dR, CFG7, a)
+    per = m9["stage_ms"]
+    dom = max(per, key=per.get)
+    ab = algorithmic_bytes(W, H, D)
+    achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
+    frame_gbs = ab["frame"] / (m9["ms"] * 1e-3) / 1e9
+    out = {
+        "metric": "dense Mpx*disp/s at 848x480x128 (C2: census 9x7, 8 BP iterations)",
+        "value": round(W * H * D / (m9["ms"] * 1e-3) / 1e6, 1),
+        "unit": "Mpx*disp/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(m9["ms"], 4),
+        "frames_per_s": round(1e3 / m9["ms"], 1),
+        "single_stream_ms_per_step": round(m9["ms_single"], 4),
+        "coverage": m9["coverage"],
+        "deterministic": m9["deterministic"] and m7["deterministic"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i16 volume / f32 messages",
+        "data": "synthetic (seeded slanted-plane pair, doubleeye_amd.synth.slanted_pair seed 2)",
+        "config": {"workload": "dense MASDA 848x480, D=128 (d=1..128), census 9x7, "
+                               "sigma_s 20, sigma_r 10/255, 8 BP iterations, top-2",
+                   "W": W, "H": H, "D": D, "iters": 8, "census": "9x7",
+                   "frames_in_flight": nfl},
+        "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": TRAFFIC.get(dom),
+                     "traffic_source": TRAFFIC_SOURCE,
+                     "kernel_ms": round(per[dom], 4),
+                     "stage_ms": {n: round(v, 4) for n, v in per.items()},
+                     "frame_algorithmic_bytes": ab["frame"],
+                     "frame_achieved_gbs": round(frame_gbs, 1),
+                     "frame_frac": round(frame_gbs / HBM_PEAK_GBS, 4)},
+        # the reference's own tool is fixed at census 7x7 (core/tools/de_dense.cpp:748):
+        # the same workload at 7x7, timed beside the reference tool
+        "census7x7": {"value": round(W * H * D / (m7["ms"] * 1e-3) / 1e6, 1),
+                      "ms_per_step": round(m7["ms"], 4),
+                      "single_stream_ms_per_step": round(m7["ms_single"], 4),
+                      "stage_ms": {n: round(v, 4) for n, v in m7["stage_ms"].items()},
+                      "coverage": m7["coverage"]},
+    }
+    for c in extra:
+        c.close()
+    ctx.close()
+    if not a.no_cpu_baseline:
+        Lc, Rc = np.ascontiguousarray(L), np.ascontiguousarray(R)
+        out["cpu_baseline"] = cpu_baseline_port(Lc, Rc, a.cpu_budget)
+        out["census7x7"]["cpu_reference"] = cpu_baseline(Lc, Rc, a.cpu_budget / 2)
+    if a.c5_seconds > 0:
+        # the second half of the metric: p50/p99 frame latency at 16 mixed streams
+        c5 = run_c5(a.c5_seconds)
+        out["c5"] = {k: c5[k] for k in ("p50_ms", "p99_ms", "max_ms", "frames_per_s",
+                                         "dense_mpx_disp_per_s", "mean_batch", "max_batch",
+                                         "frames", "dropped_busy",
+                                         "sampled_results_identical")}
+        out["c5"]["seconds"] = a.c5_seconds
+        out["p50_ms"] = c5["p50_ms"]
+        out["p99_ms"] = c5["p99_ms"]
+        out["mean_batch"] = c5["mean_batch"]
+    print(json.dumps(out), flush=True)
+
+
+def measure_c2(ctx, extra, dL, dR, cfg_kw: dict, a) -> dict:
+    """One configuration of the C2 pair: frames in flight (a captured plan per
+    frame slot, slot k on HIP stream k), then one frame at a time, then the
+    per-stage kernel times from hipEvents between the launches on the
+    launching stream (one frame at a time, so each kernel runs alone)."""
+    import numpy as np
+
+    from doubleeye_amd.dense import Cfg, Pair
+
+    nfl = len(extra) + 1
+    cfg = Cfg(**cfg_kw)
     disp = ctx.empty((H, W), np.float32)
     margin = ctx.empty((H, W), np.float32)
-    cfg = Cfg(**CFG)
 
     def step():
         ctx.run_dense_dev(dL.ptr, dR.ptr, W, H, cfg, disp.ptr, margin.ptr)
@@ -909,8 +1008,6 @@ def main():
     step()
     ctx.sync()
     first = disp.numpy()
-
-    # frames in flight: a plan per frame slot, slot k on stream k
     slots = []
     for _ in range(nfl):
         o = (ctx.empty((H, W), np.float32), ctx.empty((H, W), np.float32))
@@ -931,14 +1028,10 @@ def main():
     t0 = time.perf_counter()
     run_inflight(a.steps)
     sync_all()
-    dt = time.perf_counter() - t0
-    ms = dt * 1e3 / a.steps
-    value = W * H * D / (ms * 1e-3) / 1e6
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
     deterministic = all(bool(np.array_equal(first.view(np.uint32), o[0].numpy().view(np.uint32)))
                         for _, o in slots)
     last = slots[0][1][0].numpy()
-
-    # the same with one frame at a time, for reference
     for _ in range(a.warmup):
         step()
     ctx.sync()
@@ -949,8 +1042,6 @@ def main():
     ms_single = (time.perf_counter() - t0) * 1e3 / a.steps
     deterministic = deterministic and bool(
         np.array_equal(first.view(np.uint32), disp.numpy().view(np.uint32)))
-
-    # per-stage kernel durations, hipEvents between launches on this stream
     ctx.set_profiling(True)
     for _ in range(max(20, a.steps // 4)):
         step()
@@ -959,49 +1050,12 @@ def main():
     ctx.set_profiling(False)
     names = ["prep", "row", "col", "solve"]
     per = {n: stage_ms[i] / max(1, runs) for i, n in enumerate(names)}
-    dom = max(per, key=per.get)
-    ab = algorithmic_bytes(W, H, D)
-    # HBM bytes per launch of each kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE
-    # on this workload (profiles/dense_c2_848x480x128_kernels.md), scaled by the
-    # known-size copy/fill calibration (reads x2.00, writes x1.00)
-    traffic = {"row": 282.0e6, "col": 245.9e6, "solve": 9.9e6, "prep": 18.3e6}
-    achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
-    frame_gbs = ab["frame"] / (ms * 1e-3) / 1e9
-    out = {
-        "metric": "dense Mpx*disp/s at 848x480x128 (C2, 8 BP iterations)",
-        "value": round(value, 1),
-        "unit": "Mpx*disp/s",
-        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(ms, 4),
-        "frames_per_s": round(1e3 / ms, 1),
-        "single_stream_ms_per_step": round(ms_single, 4),
-        "coverage": round(float(np.isfinite(last).mean()), 4),
-        "deterministic": deterministic,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "i16 volume / f32 messages",
-        "data": "synthetic (seeded slanted-plane pair, doubleeye_amd.synth.slanted_pair seed 2)",
-        "config": {"workload": "dense MASDA 848x480, D=128 (d=1..128), census 7x7, "
-                               "sigma_s 20, sigma_r 10/255, 8 BP iterations, top-2",
-                   "W": W, "H": H, "D": D, "iters": 8,
-                   "frames_in_flight": nfl},
-        "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
-                     "kernel_ms": round(per[dom], 4),
-                     "stage_ms": {n: round(v, 4) for n, v in per.items()},
-                     "frame_algorithmic_bytes": ab["frame"],
-                     "frame_achieved_gbs": round(frame_gbs, 1),
-                     "frame_frac": round(frame_gbs / HBM_PEAK_GBS, 4)},
-    }
-    if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(np.ascontiguousarray(L), np.ascontiguousarray(R),
-                                           a.cpu_budget)
-    for p, _ in slots:
+    for p, o in slots:
         p.close()
-    for c in extra:
-        c.close()
-    ctx.close()
-    print(json.dumps(out), flush=True)
+        o[0].free(); o[1].free()
+    disp.free(); margin.free()
+    return {"ms": ms, "ms_single": ms_single, "stage_ms": per, "deterministic": deterministic,
+            "coverage": round(float(np.isfinite(last).mean()), 4)}
 
 
 if __name__ == "__main__":

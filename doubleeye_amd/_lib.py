@@ -187,6 +187,10 @@ SIGNATURES = {
                             C.c_double, P, P, C.POINTER(I)]),
     "de_masda_decide": (I, [P, P, P, P, P, C.c_int64, C.c_int64, C.c_int64, C.c_double, P]),
     "de_seg_max_excluding": (I, [P, P, P, C.c_int64, C.c_int64, P]),
+    "de_census_points": (I, [P, P, I, I, P, C.c_int64, I, I, P]),
+    "de_hamming": (I, [P, P, P, C.c_int64, P]),
+    "de_build_problem": (I, [P, P, P, C.c_int64, P, P, C.c_int64, I, C.c_double, C.c_double,
+                             C.c_double, C.c_double, C.c_double, P]),
     "de_dense_masda": (I, [P, P, P, I, I, I, I, I, I, C.c_double, C.c_double, I, C.c_double,
                            C.c_double, P, P, C.POINTER(I)]),
     "de_stream_packet_bytes": (C.c_size_t, [I, I, I]),

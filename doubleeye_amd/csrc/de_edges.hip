@@ -11,6 +11,8 @@
 //                       beta + rho - s, greedy decision in belief order.
 //   masda / decide      masda_stereo.py:329-414: the same on a dense score
 //                       matrix (-inf = no edge), edges in row-major order.
+//   census / hamming /  masda_stereo.py:264-326: descriptors at points,
+//   build_problem       popcount distances, the keypoint score matrix.
 //
 // Layout: the edges stay in the caller's order; two counting sorts give
 // segment lists by left index (rows) and by right index (columns). Each
@@ -508,6 +510,74 @@ using namespace de;
 namespace {
 
 // host-pointer staging: copy arrays to the context's edge io buffer
+// ---- census / hamming / build_problem (masda_stereo.py:264-326) ----
+
+// census(img, pts, half_w, half_h): one thread per point; bit b in scan order
+// dy-major, dx-minor, centre skipped, set when the neighbour is darker. Indices
+// follow numpy's: a negative row or column counts from the end (the host has
+// already refused any index numpy would refuse).
+__global__ __launch_bounds__(kT) void k_census_points(const uint8_t* __restrict__ img, int W, int H,
+                                                      const int64_t* __restrict__ xy, int64_t n,
+                                                      int hw, int hh, uint64_t* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+  if (i >= n) return;
+  const int64_t x = xy[2 * i], y = xy[2 * i + 1];
+  auto at = [&](int64_t yy, int64_t xx) -> int {
+    if (yy < 0) yy += H;
+    if (xx < 0) xx += W;
+    return img[size_t(yy) * W + size_t(xx)];
+  };
+  const int c = at(y, x);
+  uint64_t bits = 0;
+  int b = 0;
+  for (int dy = -hh; dy <= hh; ++dy)
+    for (int dx = -hw; dx <= hw; ++dx) {
+      if (dx == 0 && dy == 0) continue;
+      if (at(y + dy, x + dx) < c) bits |= uint64_t(1) << b;
+      ++b;
+    }
+  out[i] = bits;
+}
+
+// hamming(a, b): popcount of a ^ b, elementwise (broadcast on the host).
+__global__ __launch_bounds__(kT) void k_hamming(const uint64_t* __restrict__ a,
+                                                const uint64_t* __restrict__ b, int64_t n,
+                                                int32_t* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+  if (i < n) out[i] = __popcll(a[i] ^ b[i]);
+}
+
+// build_problem: S[i, j] = (BITS/2 - h)/(BITS/2) - w_y (dy/sigma_y)^2 where
+// |dy| <= max_dy and dmin <= d <= dmax, -inf elsewhere; one thread per (i, j).
+// T is the points' type: with float32 points numpy evaluates dy, d, the gate
+// and the penalty in float32 (the Python-float parameters take the array's
+// type) and desc in float64; the difference is float64, stored as float32.
+// With float64 points everything is float64 until the store.
+constexpr double kBits = 48.0;     // masda_stereo.py:302, the 7x7 census length
+template <class T>
+__global__ __launch_bounds__(kT) void k_build_problem(const T* __restrict__ pl,
+                                                      const uint64_t* __restrict__ dl, int m,
+                                                      const T* __restrict__ pr,
+                                                      const uint64_t* __restrict__ dr, int n,
+                                                      T max_dy, T dmin, T dmax, T sigma_y,
+                                                      T w_y, float* __restrict__ S) {
+  const int64_t e = int64_t(blockIdx.x) * kT + threadIdx.x;
+  if (e >= int64_t(m) * n) return;
+  const int i = int(e / n), j = int(e - int64_t(i) * n);
+  const T dy = pl[2 * i + 1] - pr[2 * j + 1];
+  const T d = pl[2 * i] - pr[2 * j];
+  const T ady = dy < T(0) ? -dy : dy;
+  float v = -INFINITY;
+  if (ady <= max_dy && d >= dmin && d <= dmax) {
+    const int h = __popcll(dl[i] ^ dr[j]);
+    const double desc = (kBits / 2 - double(h)) / (kBits / 2);
+    const T q = dy / sigma_y;
+    const T pen = w_y * (q * q);
+    v = float(desc - double(pen));
+  }
+  S[e] = v;
+}
+
 struct Staging {
   uint8_t* p;
   void* put(const void* h, size_t bytes, hipStream_t s) {
@@ -845,6 +915,96 @@ int de_dense_masda(de_ctx* c, const uint8_t* L, const uint8_t* R, int W, int H, 
   if (n_iter) *n_iter = state[2];
   if (hist && state[2] > 0)
     DE_HIP(hipMemcpy(hist, s.hist, size_t(state[2]) * 8, hipMemcpyDeviceToHost));
+  return DE_OK;
+}
+
+int de_census_points(de_ctx* c, const uint8_t* img, int W, int H, const int64_t* xy,
+                     int64_t n, int hw, int hh, uint64_t* out) {
+  if (!c || !img || (n > 0 && (!xy || !out))) return fail(DE_ERR_ARG, "de_census_points: null argument");
+  if (W < 1 || H < 1 || n < 0 || n > (1ll << 31) || hw < 0 || hh < 0)
+    return fail(DE_ERR_ARG, "census: image %dx%d, %lld points, window %dx%d", W, H, (long long)n,
+                2 * hw + 1, 2 * hh + 1);
+  if ((2 * hw + 1) * (2 * hh + 1) - 1 > 64)
+    return fail(DE_ERR_UNSUPPORTED, "census: a %dx%d window has more than 64 bits", 2 * hw + 1,
+                2 * hh + 1);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t x = xy[2 * i], y = xy[2 * i + 1];
+    if (x - hw < -int64_t(W) || x + hw >= W || y - hh < -int64_t(H) || y + hh >= H)
+      return fail(DE_ERR_ARG, "census: point %lld (%lld, %lld) indexes outside the %dx%d image",
+                  (long long)i, (long long)x, (long long)y, W, H);
+  }
+  if (n == 0) return DE_OK;
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  Ctx& x = c->c;
+  DE_HIP(hipSetDevice(x.device));
+  const size_t WH = size_t(W) * H;
+  if (int rc = x.ed_io.ensure(a256(WH) + a256(size_t(n) * 16) + a256(size_t(n) * 8))) return rc;
+  Staging g{static_cast<uint8_t*>(x.ed_io.ptr)};
+  const uint8_t* dimg = static_cast<const uint8_t*>(g.put(img, WH, x.stream));
+  const int64_t* dxy = static_cast<const int64_t*>(g.put(xy, size_t(n) * 16, x.stream));
+  uint64_t* dout = static_cast<uint64_t*>(g.take(size_t(n) * 8));
+  hipLaunchKernelGGL(k_census_points, dim3(gblocks(n)), dim3(kT), 0, x.stream, dimg, W, H, dxy, n,
+                     hw, hh, dout);
+  DE_HIP(hipGetLastError());
+  DE_HIP(hipMemcpyAsync(out, dout, size_t(n) * 8, hipMemcpyDeviceToHost, x.stream));
+  DE_HIP(hipStreamSynchronize(x.stream));
+  return DE_OK;
+}
+
+int de_hamming(de_ctx* c, const uint64_t* a, const uint64_t* b, int64_t n, int32_t* out) {
+  if (!c || (n > 0 && (!a || !b || !out))) return fail(DE_ERR_ARG, "de_hamming: null argument");
+  if (n < 0 || n > (1ll << 32)) return fail(DE_ERR_ARG, "hamming: %lld elements", (long long)n);
+  if (n == 0) return DE_OK;
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  Ctx& x = c->c;
+  DE_HIP(hipSetDevice(x.device));
+  if (int rc = x.ed_io.ensure(a256(size_t(n) * 8) * 2 + a256(size_t(n) * 4))) return rc;
+  Staging g{static_cast<uint8_t*>(x.ed_io.ptr)};
+  const uint64_t* da = static_cast<const uint64_t*>(g.put(a, size_t(n) * 8, x.stream));
+  const uint64_t* db = static_cast<const uint64_t*>(g.put(b, size_t(n) * 8, x.stream));
+  int32_t* dout = static_cast<int32_t*>(g.take(size_t(n) * 4));
+  hipLaunchKernelGGL(k_hamming, dim3(gblocks(n)), dim3(kT), 0, x.stream, da, db, n, dout);
+  DE_HIP(hipGetLastError());
+  DE_HIP(hipMemcpyAsync(out, dout, size_t(n) * 4, hipMemcpyDeviceToHost, x.stream));
+  DE_HIP(hipStreamSynchronize(x.stream));
+  return DE_OK;
+}
+
+int de_build_problem(de_ctx* c, const void* pl, const uint64_t* dl, int64_t m, const void* pr,
+                     const uint64_t* dr, int64_t n, int fp64, double max_dy, double dmin,
+                     double dmax, double sigma_y, double w_y, float* S) {
+  if (!c || (m > 0 && (!pl || !dl)) || (n > 0 && (!pr || !dr)) || (m > 0 && n > 0 && !S))
+    return fail(DE_ERR_ARG, "de_build_problem: null argument");
+  if (m < 0 || n < 0 || m > (1 << 30) || n > (1 << 30) || m * n > (1ll << 32))
+    return fail(DE_ERR_ARG, "build_problem: %lld x %lld", (long long)m, (long long)n);
+  if (m == 0 || n == 0) return DE_OK;
+  std::lock_guard<std::mutex> lk(c->c.mu);
+  Ctx& x = c->c;
+  DE_HIP(hipSetDevice(x.device));
+  const size_t ts = fp64 ? 8 : 4;
+  const size_t mn = size_t(m) * size_t(n);
+  if (int rc = x.ed_io.ensure(a256(size_t(m) * 2 * ts) + a256(size_t(n) * 2 * ts) +
+                              a256(size_t(m) * 8) + a256(size_t(n) * 8) + a256(mn * 4)))
+    return rc;
+  Staging g{static_cast<uint8_t*>(x.ed_io.ptr)};
+  const void* dpl = g.put(pl, size_t(m) * 2 * ts, x.stream);
+  const void* dpr = g.put(pr, size_t(n) * 2 * ts, x.stream);
+  const uint64_t* ddl = static_cast<const uint64_t*>(g.put(dl, size_t(m) * 8, x.stream));
+  const uint64_t* ddr = static_cast<const uint64_t*>(g.put(dr, size_t(n) * 8, x.stream));
+  float* dS = static_cast<float*>(g.take(mn * 4));
+  if (fp64)
+    hipLaunchKernelGGL(k_build_problem<double>, dim3(gblocks((long long)mn)), dim3(kT), 0,
+                       x.stream, static_cast<const double*>(dpl), ddl, int(m),
+                       static_cast<const double*>(dpr), ddr, int(n), max_dy, dmin, dmax, sigma_y,
+                       w_y, dS);
+  else
+    hipLaunchKernelGGL(k_build_problem<float>, dim3(gblocks((long long)mn)), dim3(kT), 0,
+                       x.stream, static_cast<const float*>(dpl), ddl, int(m),
+                       static_cast<const float*>(dpr), ddr, int(n), float(max_dy), float(dmin),
+                       float(dmax), float(sigma_y), float(w_y), dS);
+  DE_HIP(hipGetLastError());
+  DE_HIP(hipMemcpyAsync(S, dS, mn * 4, hipMemcpyDeviceToHost, x.stream));
+  DE_HIP(hipStreamSynchronize(x.stream));
   return DE_OK;
 }
 

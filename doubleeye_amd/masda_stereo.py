@@ -14,6 +14,12 @@ Same names, arguments and return types:
   ``(ei, ej, se, n_nodes)`` (dense_masda.py:39)
 * ``dense_masda(left, right, dmin, dmax, ...)`` -> disparity map, the device
   pipeline of dense_masda.run (dense_masda.py:82) without its margin gate.
+* ``census(img, pts, half_w, half_h)`` -> uint64 descriptor per point
+  (masda_stereo.py:264)
+* ``hamming(a, b)`` -> int32 popcount of a ^ b, broadcast (masda_stereo.py:289)
+* ``build_problem(pl, dl, pr, dr, max_dy, dmin, dmax, sigma_y, w_y)`` ->
+  float32 m x n score matrix, -inf where no candidate (masda_stereo.py:305),
+  with the reference's 48-bit scaling (``BITS``).
 
 The returned dicts hold the same pairs as the reference's; they are built in
 ascending left index, where the reference inserts in belief order. Exactly
@@ -29,7 +35,9 @@ import numpy as np
 from ._lib import check, lib
 
 __all__ = ["masda_sparse", "masda", "decide", "_seg_max_excluding", "dense_problem",
-           "dense_masda"]
+           "dense_masda", "census", "hamming", "build_problem", "BITS"]
+
+BITS = 48   # descriptor length the score scale assumes (masda_stereo.py:302)
 
 
 def _ctx(ctx):
@@ -131,3 +139,64 @@ def dense_masda(left, right, dmin, dmax, lam=-0.1, gam=-0.1, iters=12, damping=0
                                census_cfg[0], census_cfg[1], float(lam), float(gam), int(iters),
                                float(damping), float(eps), _p(disp), _p(hist), C.byref(nit)))
     return disp, hist[: nit.value].copy()
+
+
+def census(img, pts, half_w=3, half_h=3, ctx=None):
+    """Census descriptor at each point: bit b in scan order (dy-major, dx-minor,
+    centre skipped) set when that neighbour is darker than the centre. Points
+    are (x, y), truncated to integers as ``pts.astype(int)`` does; indexing
+    follows numpy's (a negative index counts from the end; past the end raises
+    IndexError)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    H, W = img.shape
+    xy = np.ascontiguousarray(np.asarray(pts).astype(int), np.int64).reshape(-1, 2)
+    n = len(xy)
+    out = np.zeros(n, np.uint64)
+    if n == 0:
+        return out
+    hw, hh = int(half_w), int(half_h)
+    bad = ((xy[:, 0] - hw < -W) | (xy[:, 0] + hw >= W) | (xy[:, 1] - hh < -H) |
+           (xy[:, 1] + hh >= H))
+    if bad.any():
+        i = int(np.nonzero(bad)[0][0])
+        raise IndexError(f"census window of point {i} {tuple(xy[i])} is outside the "
+                         f"{H}x{W} image")
+    check(lib().de_census_points(_ctx(ctx).h, _p(img), W, H, _p(xy), n, hw, hh, _p(out)))
+    return out
+
+
+def hamming(a, b, ctx=None):
+    """popcount(a ^ b) elementwise over the broadcast shape, int32."""
+    a, b = np.broadcast_arrays(np.asarray(a, np.uint64), np.asarray(b, np.uint64))
+    shape = a.shape
+    a = np.ascontiguousarray(a).ravel()
+    b = np.ascontiguousarray(b).ravel()
+    out = np.zeros(a.size, np.int32)
+    if a.size:
+        check(lib().de_hamming(_ctx(ctx).h, _p(a), _p(b), a.size, _p(out)))
+    return out.reshape(shape)
+
+
+def build_problem(pl, dl, pr, dr, max_dy=1.5, dmin=1.0, dmax=60.0, sigma_y=1.0, w_y=1.0,
+                  ctx=None):
+    """Sparse candidate graph as a score matrix: S[i, j] = (BITS/2 - h)/(BITS/2)
+    - w_y (dy/sigma_y)^2 for |dy| <= max_dy and dmin <= d <= dmax, -inf
+    elsewhere (float32, m x n). float32 points are evaluated in numpy's float32
+    arithmetic, others in float64."""
+    pl = np.asarray(pl)
+    pr = np.asarray(pr)
+    fp64 = not (pl.dtype == np.float32 and pr.dtype == np.float32)
+    t = np.float64 if fp64 else np.float32
+    pl = np.ascontiguousarray(pl, t).reshape(-1, 2)
+    pr = np.ascontiguousarray(pr, t).reshape(-1, 2)
+    dl = np.ascontiguousarray(dl, np.uint64).ravel()
+    dr = np.ascontiguousarray(dr, np.uint64).ravel()
+    m, n = len(pl), len(pr)
+    if len(dl) != m or len(dr) != n:
+        raise ValueError("one descriptor per point")
+    S = np.full((m, n), -np.inf, np.float32)
+    if m and n:
+        check(lib().de_build_problem(_ctx(ctx).h, _p(pl), _p(dl), m, _p(pr), _p(dr), n,
+                                     int(fp64), float(max_dy), float(dmin), float(dmax),
+                                     float(sigma_y), float(w_y), _p(S)))
+    return S

@@ -495,6 +495,27 @@ int de_masda_decide(de_ctx* ctx, const int64_t* ei, const int64_t* ej, const dou
 int de_seg_max_excluding(de_ctx* ctx, const double* vals, const int64_t* idx, int64_t n_vals,
                          int64_t n_segments, double* out);
 
+/* census(img, pts, half_w, half_h) (masda_stereo.py:264-283): out[i] = the
+ * census descriptor at (xy[2i], xy[2i+1]) (integer pixel positions; a negative
+ * index counts from the end, as numpy indexing does). Refuses a point whose
+ * window reaches outside [-W, W) x [-H, H) (numpy raises there) and windows
+ * of more than 64 bits. */
+int de_census_points(de_ctx* ctx, const uint8_t* img, int W, int H, const int64_t* xy,
+                     int64_t n_points, int half_w, int half_h, uint64_t* out);
+
+/* hamming(a, b) (masda_stereo.py:289-294): out[i] = popcount(a[i] ^ b[i]). */
+int de_hamming(de_ctx* ctx, const uint64_t* a, const uint64_t* b, int64_t n, int32_t* out);
+
+/* build_problem (masda_stereo.py:305-326): the m x n score matrix S, row-major,
+ * S[i][j] = (24 - Hamming(dl[i], dr[j])) / 24 - w_y (dy / sigma_y)^2 where
+ * |dy| <= max_dy and dmin <= d <= dmax (dy, d: left minus right y, x), -inf
+ * elsewhere. pl, pr: (x, y) pairs, float32 (fp64 = 0) or float64 (fp64 = 1);
+ * the arithmetic follows numpy's types for each (the parameters take the
+ * points' type). */
+int de_build_problem(de_ctx* ctx, const void* pl, const uint64_t* dl, int64_t m, const void* pr,
+                     const uint64_t* dr, int64_t n, int fp64, double max_dy, double dmin,
+                     double dmax, double sigma_y, double w_y, float* S);
+
 /* dense_problem + masda_sparse + the disparity map of dense_masda.run
  * (dense_masda.py:82-110, without the optional margin gate), all on the
  * device: disp[y*W + x] = x - (j mod W) for matched pixels, NaN elsewhere. */

@@ -59,6 +59,7 @@ def lib():
         L.orc_confidence.restype = C.c_float
         L.orc_confidence_u8.argtypes = [C.c_float] * 5
         L.orc_confidence_u8.restype = C.c_uint8
+        L.orc_dedd_planes.argtypes = [P, P, P, P, C.c_int, C.c_float, P, P, P]
         for name in ("orc_sparse_match_masda", "orc_masda_edges", "orc_knn_bp"):
             if hasattr(L, name):
                 getattr(L, name).restype = C.c_int
@@ -216,3 +217,20 @@ def confidence(s1, s2, lam, lrc=0.0, tol=1.0) -> float:
 
 def confidence_u8(s1, s2, lam, lrc=0.0, tol=1.0) -> int:
     return int(lib().orc_confidence_u8(s1, s2, lam, lrc, tol))
+
+
+def dedd_planes(disp, cand6, lam: float = -0.1):
+    """Q4 disparity, u8 confidence and the float P*255 it rounds, from a
+    frame's disparity and --out-conf planes (s1 = plane 0, s2 = plane 1, the
+    solver's reverse-match distance = plane 5), as the reference's stream
+    output computes them."""
+    disp = np.ascontiguousarray(disp, np.float32)
+    c6 = np.ascontiguousarray(cand6, np.float32)
+    n = disp.size
+    q4 = np.empty(disp.shape, np.int16)
+    conf = np.empty(disp.shape, np.uint8)
+    p255 = np.empty(disp.shape, np.float32)
+    s1, s2, lrc = (np.ascontiguousarray(c6[i]) for i in (0, 1, 5))
+    lib().orc_dedd_planes(_p(disp), _p(s1), _p(s2), _p(lrc), n, lam, _p(q4), _p(conf),
+                          _p(p255))
+    return q4, conf, p255

@@ -908,4 +908,20 @@ uint8_t orc_confidence_u8(float s1, float s2, float lambda, float lrc, float lrc
   return uint8_t(std::lround(orc_confidence(s1, s2, lambda, lrc, lrc_tol) * 255.f));
 }
 
+// The DEDD output planes of one frame (core/tools/de_dense_cuda.cu:731-745):
+// Q4 disparity lround(16 d) where d > 0 and finite, else -32768; u8 confidence
+// of (s1, s2, lambda, lrc if finite else 0), 0 where Q4 is -32768. p255 (nullable)
+// receives the float P * 255 the byte rounds, 0 where there is no answer.
+void orc_dedd_planes(const float* disp, const float* s1, const float* s2, const float* lrc,
+                     int n, float lambda, int16_t* q4, uint8_t* conf, float* p255) {
+  for (int i = 0; i < n; ++i) {
+    const float d = disp[i];
+    q4[i] = (d > 0.f && std::isfinite(d)) ? int16_t(std::lround(d * 16.0f)) : int16_t(-32768);
+    const float l = std::isfinite(lrc[i]) ? lrc[i] : 0.f;
+    const float p = orc_confidence(s1[i], s2[i], lambda, l, 1.f) * 255.f;
+    conf[i] = q4[i] == int16_t(-32768) ? uint8_t(0) : uint8_t(std::lround(p));
+    if (p255) p255[i] = q4[i] == int16_t(-32768) ? 0.f : p;
+  }
+}
+
 }  // extern "C"

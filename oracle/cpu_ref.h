@@ -38,6 +38,8 @@ int orc_volume_f32(const uint8_t* L, const uint8_t* R, int W, int H,
                    const OrcCfg* cfg, float* vol);
 float orc_confidence(float s1, float s2, float lambda, float lrc, float lrc_tol);
 uint8_t orc_confidence_u8(float s1, float s2, float lambda, float lrc, float lrc_tol);
+void orc_dedd_planes(const float* disp, const float* s1, const float* s2, const float* lrc,
+                     int n, float lambda, int16_t* q4, uint8_t* conf, float* p255);
 
 /* --prior / --c2f (core/tools/de_dense.cpp:797-853, 1156-1296, 1889-1966):
  * the prior as a mask on absolute-disparity planes; sub-pixel off. prior is

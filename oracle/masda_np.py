@@ -10,6 +10,9 @@ Follows article/masda_stereo.py and article/dense_masda.py:
   decide              masda_stereo.py:392-414
   dense_problem       dense_masda.py:39-79 (census from the compiled oracle,
                       equal to the rolled-array census on interior pixels)
+  census_points       masda_stereo.py:264-283
+  hamming             masda_stereo.py:286-294
+  build_problem       masda_stereo.py:302-326
 `canonical=False` orders the greedy decision with numpy's default argsort,
 as the reference does; `canonical=True` breaks exact belief ties by edge
 index (a stable sort), the order the HIP path uses.
@@ -147,3 +150,56 @@ def dense_problem(left, right, dmin, dmax, census_cfg=(3, 3)):
     if not ei:
         return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0, np.float32), H * W
     return np.concatenate(ei), np.concatenate(ej), np.concatenate(se), H * W
+
+
+# ---- census / hamming / build_problem (masda_stereo.py:264-326) ----
+
+BITS = 48   # masda_stereo.py:302
+
+
+def census_points(img, pts, half_w=3, half_h=3):
+    """masda_stereo.py:264-283: a bit per neighbour in scan order, set when it
+    is darker than the centre, numpy indexing (negative wraps)."""
+    out = np.zeros(len(pts), np.uint64)
+    f = np.asarray(img).astype(np.int32)
+    for n, (x, y) in enumerate(np.asarray(pts).astype(int)):
+        c = f[y, x]
+        bits, b = 0, 0
+        for dy in range(-half_h, half_h + 1):
+            for dx in range(-half_w, half_w + 1):
+                if dx == 0 and dy == 0:
+                    continue
+                if f[y + dy, x + dx] < c:
+                    bits |= 1 << b
+                b += 1
+        out[n] = np.uint64(bits)
+    return out
+
+
+_POPCNT = np.array([bin(i).count("1") for i in range(256)], np.int32)
+
+
+def hamming(a, b):
+    """masda_stereo.py:289-294: byte-table popcount of a ^ b, int32."""
+    x = np.bitwise_xor(np.asarray(a, np.uint64), np.asarray(b, np.uint64))
+    total = np.zeros(x.shape, np.int32)
+    for shift in range(0, 64, 8):
+        total += _POPCNT[((x >> np.uint64(shift)) & np.uint64(0xFF)).astype(np.uint8)]
+    return total
+
+
+def build_problem(pl, dl, pr, dr, max_dy=1.5, dmin=1.0, dmax=60.0, sigma_y=1.0, w_y=1.0):
+    """masda_stereo.py:305-326, row by row, numpy's own type rules."""
+    m, n = len(pl), len(pr)
+    S = np.full((m, n), -np.inf, np.float32)
+    for i in range(m):
+        dy = pl[i, 1] - pr[:, 1]
+        d = pl[i, 0] - pr[:, 0]
+        ok = (np.abs(dy) <= max_dy) & (d >= dmin) & (d <= dmax)
+        if not ok.any():
+            continue
+        h = hamming(np.full(ok.sum(), dl[i], np.uint64), dr[ok])
+        desc = (BITS / 2 - h) / (BITS / 2)
+        pen = w_y * (dy[ok] / sigma_y) ** 2
+        S[i, ok] = desc - pen
+    return S

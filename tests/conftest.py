@@ -1,6 +1,7 @@
 import os
 import sys
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -47,3 +48,25 @@ def flags_to_kwargs(flags: str) -> dict:
         elif a == "--c2f": kw["c2f"] = 1; i += 1
         else: raise ValueError(a)
     return kw
+
+
+CONF_BOUNDARY = 2e-3   # |P*255 - (k + 0.5)| where a confidence byte may round the other way
+CONF_MAX_FRAC = 1e-3   # differing confidence bytes per answered pixel
+
+
+def check_conf_bytes(conf, oconf, p255, answered, tag):
+    """u8 confidence bytes against the oracle's: the device's logf/expf and
+    glibc's differ by about an ulp on some arguments, so a byte may round the
+    other way where P*255 lies next to a .5 boundary, and nowhere else.
+    Returns the number of differing bytes."""
+    diff = conf.astype(np.int32) - oconf.astype(np.int32)
+    bad = diff != 0
+    n = int(bad.sum())
+    dist = np.abs(p255 - (np.floor(p255) + np.float32(0.5)))
+    worst = float(dist[bad].max()) if n else 0.0
+    print(f"[conf] {tag}: {n} of {int(answered.sum())} answered bytes differ by one "
+          f"(largest distance of P*255 from k+0.5 among them {worst:.2e})", flush=True)
+    assert (np.abs(diff) <= 1).all(), f"{tag}: a byte differs by more than one"
+    assert (dist[bad] <= CONF_BOUNDARY).all(), f"{tag}: a byte differs away from a .5 boundary"
+    assert n <= max(1, CONF_MAX_FRAC * answered.sum()), f"{tag}: {n} differing bytes"
+    return n

@@ -117,6 +117,25 @@ def test_c2_full_size(ctx, oracle):
     check_dense(ctx, oracle, L, R, kw, "C2 848x480x128")
 
 
+def test_c2_full_size_census_9x7(ctx, oracle):
+    """C2 as stated: 848x480, 128 disparities, 8 iterations, census 9x7
+    (62 bits, score level (31 - h)/31, x-border 4)."""
+    from doubleeye_amd import synth
+    L, R, _ = synth.slanted_pair(848, 480, seed=2)
+    kw = dict(dmax=128, iters=8, sigma_s=20.0, sigma_r=0.0392, census_hw=4, census_hh=3)
+    check_dense(ctx, oracle, L, R, kw, "C2 848x480x128 census 9x7")
+
+
+def test_c1_census_9x7(ctx, oracle):
+    """C1 as stated: 424x240 random-dot pair on fronto planes at 8/20/35 px,
+    48 disparities, 5 iterations, census 9x7."""
+    from doubleeye_amd import synth
+    L, R, _ = synth.fronto_pair(424, 240, seed=1, disparities=(8, 20, 35))
+    kw = dict(dmax=48, iters=5, sigma_s=20.0, sigma_r=0.0392, census_hw=4, census_hh=3)
+    d, od, ties = check_dense(ctx, oracle, L, R, kw, "C1 424x240x48 census 9x7")
+    assert np.isfinite(d).mean() > 0.5
+
+
 def test_9x7_extension_and_dmin0(ctx, oracle):
     from doubleeye_amd import synth
     L, R, _ = synth.fronto_pair(300, 160, seed=9)

@@ -90,3 +90,50 @@ def test_out_of_range_indices_raise(ctx):
     from doubleeye_amd._lib import DeError
     with pytest.raises(DeError):
         gm.masda_sparse(np.array([0, 5]), np.array([0, 0]), np.array([0.5, 0.5]), 3, 3, ctx=ctx)
+
+
+def test_census_hamming_build_problem_match_oracle(ctx):
+    """census(img, pts, ...), hamming(a, b) and build_problem(...) on the
+    device equal the oracle's restatement of masda_stereo.py:264-326 bit for
+    bit: descriptors, distances, and every score (float32 and float64
+    points, parameters on the gate boundaries)."""
+    from doubleeye_amd import masda_stereo as gm
+    from doubleeye_amd import synth
+    from oracle import masda_np as mn
+    L, R, _ = synth.fronto_pair(300, 120, seed=31)
+    rng = np.random.default_rng(31)
+    for hw, hh in ((3, 3), (4, 3), (2, 1)):
+        pts = np.stack([rng.uniform(hw, 300 - hw, 500), rng.uniform(hh, 120 - hh, 500)], 1)
+        pts = pts.astype(np.float32)
+        pts[:4] = [[0, 0], [-3, -2], [299 - hw, 119 - hh], [hw, hh]]       # numpy wrap-around
+        note(f"[np-shapes] census {2*hw+1}x{2*hh+1}")
+        got = gm.census(L, pts, hw, hh, ctx=ctx)
+        assert got.dtype == np.uint64
+        assert np.array_equal(got, mn.census_points(L, pts, hw, hh))
+    with pytest.raises(IndexError):
+        gm.census(L, np.array([[297.0, 5.0]], np.float32), 3, 3, ctx=ctx)
+    a = rng.integers(0, 2 ** 63, (70, 1), dtype=np.uint64)
+    b = rng.integers(0, 2 ** 63, (1, 90), dtype=np.uint64)
+    h = gm.hamming(a, b, ctx=ctx)
+    assert h.dtype == np.int32 and h.shape == (70, 90)
+    assert np.array_equal(h, mn.hamming(a, b))
+    # keypoint-like problem: left/right points on the same rows, descriptors
+    # from the census, plus points on the gate boundaries
+    m, n = 700, 650
+    pl = np.stack([rng.uniform(10, 290, m), rng.integers(5, 115, m) + rng.choice([0, .25, .5], m)], 1)
+    pr = np.stack([rng.uniform(10, 290, n), rng.integers(5, 115, n) + rng.choice([0, .25, .5], n)], 1)
+    pr[:40] = pl[:40] - [[1.0, 1.5]] * 40                                # d = 1, dy = 1.5
+    pr[40:80] = pl[40:80] - [[60.0, -1.5]] * 40                          # d = 60, dy = -1.5
+    pr[80:120] = pl[80:120] - [[60.5, 0.0]] * 40                         # just outside
+    for t in (np.float32, np.float64):
+        plt, prt = pl.astype(t), pr.astype(t)
+        dl = mn.census_points(L, plt, 3, 3)
+        dr = mn.census_points(R, prt, 3, 3)
+        for kw in (dict(), dict(max_dy=0.5, dmin=2.0, dmax=40.0, sigma_y=0.7, w_y=0.3),
+                   dict(sigma_y=3.0, w_y=1.7)):
+            note(f"[np-shapes] build_problem {t.__name__} {kw}")
+            got = gm.build_problem(plt, dl, prt, dr, ctx=ctx, **kw)
+            want = mn.build_problem(plt, dl, prt, dr, **kw)
+            assert got.dtype == np.float32 and got.shape == (m, n)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (t, kw)
+            assert np.isfinite(got).sum() > 100

@@ -202,3 +202,92 @@ def test_packed_top2_butterfly_semantics():
                    ((p1[0] >> 8) - 1 - 16384) if p1[0] else -32768,
                    (255 - (p1[0] & 255)) if p1[0] else -1)
             assert got == (b0, i0, b1, i1)
+
+
+def _brute_census(img, y, x, hw, hh):
+    """Census bits by enumeration: bit b in scan order dy-major, dx-minor,
+    centre skipped, set when the neighbour is darker than the centre."""
+    v, b = 0, 0
+    for dy in range(-hh, hh + 1):
+        for dx in range(-hw, hw + 1):
+            if dx == 0 and dy == 0:
+                continue
+            if img[y + dy, x + dx] < img[y, x]:
+                v |= 1 << b
+            b += 1
+    return v, b
+
+
+def test_score_table_9x7_closed_forms(oracle):
+    """The 9x7 window has 62 bits, so the census level is (31 - h) / 31 in Q14
+    truncated toward zero: +1 at h = 0, 0 at h = 31, -1 at h = 62, strictly
+    decreasing, antisymmetric about h = 31, within one unit of the exact
+    rational. The reference hard-codes the 48-bit table (half = 24,
+    de_dense.cpp:1201-1210); a table built with the wrong half fails here."""
+    nbits = 9 * 7 - 1
+    half = nbits // 2
+    assert half == 31
+    tbl, adt, wq = oracle.score_tables(half, 0.15, 10)
+    t = tbl[:nbits + 1].astype(np.int64)
+    assert t[0] == 16384 and t[half] == 0 and t[nbits] == -16384
+    assert (np.diff(t) < 0).all()
+    assert np.array_equal(t, -t[::-1])
+    exact = (half - np.arange(nbits + 1)) * 16384 / half
+    assert (np.abs(t - exact) < 1).all() and (np.abs(t) <= np.abs(exact)).all()
+    # the 7x7 table has the same shape at half = 24
+    t7, _, _ = oracle.score_tables(24, 0.15, 10)
+    assert t7[0] == 16384 and t7[24] == 0 and t7[48] == -16384
+    # truncated-AD level: +1 at |dI| = 0, -1 from T on; blend weight int(0.15 * 1024)
+    assert adt[0] == 16384 and (adt[10:] == -16384).all() and (np.diff(adt[:11]) < 0).all()
+    assert wq == 153
+
+
+def test_graded_score_9x7_brute_force(oracle):
+    """Raw graded scores of the 9x7 path (a zero filter coefficient, sigma_s ->
+    0, leaves every plane unfiltered) against a brute-force census, Hamming
+    distance and blend at every pixel, and the valid window y in [3, H-3),
+    x in [4 + d, W - 4) with zero outside (de_dense.cpp:1167-1243 with the
+    window's own border)."""
+    from doubleeye_amd import synth
+    L, R, _ = synth.fronto_pair(60, 24, seed=3, disparities=(2, 5, 9))
+    H, W = L.shape
+    hw, hh, dmin, dmax = 4, 3, 1, 12
+    assert not oracle.rf_table(0.01, 0.0392).any()      # a = 0: f = c at every step
+    planes = oracle.filtered_planes(L, R, oracle.cfg(dmin=dmin, dmax=dmax, sigma_s=0.01,
+                                                     census_hw=hw, census_hh=hh), 0, dmax)
+    half, wq, T = 31, 153, 10
+    cl = {}
+    cr = {}
+    for y in range(hh, H - hh):
+        for x in range(hw, W - hw):
+            cl[y, x], nb = _brute_census(L.astype(int), y, x, hw, hh)
+            cr[y, x], _ = _brute_census(R.astype(int), y, x, hw, hh)
+    assert nb == 62
+    for k in range(dmax):
+        d = dmin + k
+        want = np.zeros((H, W), np.int64)
+        for y in range(hh, H - hh):
+            for x in range(hw + d, W - hw):
+                h = bin(cl[y, x] ^ cr[y, x - d]).count("1")
+                num = (half - h) * 16384
+                c = abs(num) // half * (1 if num >= 0 else -1)      # C++ truncation
+                a = 16384 - (2 * 16384 * min(abs(int(L[y, x]) - int(R[y, x - d])), T)) // T
+                want[y, x] = (c * (1024 - wq) + a * wq) >> 10
+        np.testing.assert_array_equal(planes[k], want, err_msg=f"plane d={d}")
+    assert (planes[:, :, :hw + dmin] == 0).all() and (planes[:, :, W - hw:] == 0).all()
+
+
+def test_dedd_planes_oracle(oracle):
+    """Q4 = lround(16 d) for d > 0 finite, -32768 otherwise; confidence 0 where
+    Q4 is none, else confidence_u8 with NaN lrc read as 0
+    (de_dense_cuda.cu:731-745); goldens of test_confidence.cpp:52-56, 117-119."""
+    disp = np.array([1.03125, 2.0, -1.0, np.nan, 0.0, 7.96875, 3.5], np.float32)
+    c6 = np.full((6, disp.size), -1e30, np.float32)
+    c6[0] = [0.9, 0.5, 0.9, 0.9, 0.9, 0.682, 0.9]
+    c6[1] = [0.3, 0.49, 0.3, 0.3, 0.3, 0.537, 0.3]
+    c6[5] = [np.nan, 0.0, 0.0, 0.0, 0.0, np.nan, 9.0]
+    q4, conf, p255 = oracle.dedd_planes(disp, c6, -0.1)
+    assert q4.tolist() == [17, 32, -32768, -32768, -32768, 128, 56]
+    assert conf.tolist()[:5] == [255, 179, 0, 0, 0]
+    assert conf[5] == round(0.975261 * 255) and conf[6] == round(0.35 * 255)
+    assert p255[2] == 0 and abs(p255[1] - 0.700536 * 255) < 1e-3

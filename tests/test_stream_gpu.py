@@ -8,7 +8,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import ROOT
+from conftest import ROOT, check_conf_bytes
 
 pytestmark = pytest.mark.gpu
 W, H = 200, 96
@@ -68,7 +68,11 @@ def packets_and_model():
     return data, out, dropped
 
 
-def check_packets(ctx, got, want):
+def check_packets(ctx, got, want, oracle=None):
+    """Every packet equals the plain call; with the oracle, its Q4 and
+    confidence planes also equal the oracle's DEDD planes (de_dense_cuda.cu:
+    731-745) away from exact-tie pixels, confidence bytes to the .5-boundary
+    rule."""
     from doubleeye_amd.dense import Cfg
     assert [p["frame"] for p in got] == [w[0] for w in want]
     for p, (fr, L, R) in zip(got, want):
@@ -76,9 +80,18 @@ def check_packets(ctx, got, want):
         assert np.array_equal(p["left"], L), fr
         assert np.array_equal(p["q4"], q4), fr
         assert p["conf"] is not None and np.array_equal(p["conf"], conf), fr
+        if oracle is not None:
+            od, _, oc, ties = oracle.dense_run(L, R, oracle.cfg(**KW), want_cand=True,
+                                               want_ties=True)
+            oq4, oconf, p255 = oracle.dedd_planes(od, oc, -0.1)
+            keep = ~ties.astype(bool)
+            assert np.array_equal(p["q4"][keep], oq4[keep]), fr
+            check_conf_bytes(np.where(keep, p["conf"], 0), np.where(keep, oconf, 0),
+                             np.where(keep, p255, 0), (p["q4"] != -32768) & keep,
+                             f"DEDD frame {fr}")
 
 
-def test_feed_in_chunks(ctx):
+def test_feed_in_chunks(ctx, oracle):
     from doubleeye_amd.dense import Cfg
     from doubleeye_amd.stream import Stream, parse_dedd
     data, want, dropped = packets_and_model()
@@ -91,7 +104,7 @@ def test_feed_in_chunks(ctx):
         out += s.feed(data[pos:pos + n])
         pos += n
     note(f"[stream] {len(want)} pairs")
-    check_packets(ctx, parse_dedd(out), want)
+    check_packets(ctx, parse_dedd(out), want, oracle)
     assert s.stats() == (len(want), dropped)
     s.close()
 
