@@ -149,6 +149,8 @@ SIGNATURES = {
     "de_census_dev": (I, [P, P, I, I, I, I, P, P]),
     "de_dense_run": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, P, P]),
     "de_dense_run_dev": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, P, P, P]),
+    "de_dense_run_dev_noalloc": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, P, P, P]),
+    "de_dense_noalloc_limit": (C.c_size_t, []),
     "de_dense_volume": (I, [P, P, P, I, I, C.POINTER(DeCfgC), I, I, P]),
     "de_dense_candidates": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P]),
     "de_ctx_set_profiling": (I, [P, I]),

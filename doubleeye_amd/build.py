@@ -37,6 +37,7 @@ SOURCES = {
     "de_stream.hip": [],
     "de_detect.hip": ["-ffp-contract=off"],
     "de_prior.hip": [],
+    "de_noalloc.hip": [],
 }
 
 

@@ -166,6 +166,15 @@ int Ctx::frame_desc(const de_cfg& c, int W, int H, const uint8_t* L, const uint8
   const uint16_t *rf, *sc;
   int shift;
   if (int rc = tables_for(c, &rf, &sc, &shift)) return rc;
+  fill_frame_desc(c, W, H, L, R, disp, margin, cand6, conf, q4, sp, rf, sc, store_vol, fd);
+  return DE_OK;
+}
+
+// The descriptor itself, given the frame's device tables (no device calls).
+void fill_frame_desc(const de_cfg& c, int W, int H, const uint8_t* L, const uint8_t* R,
+                     float* disp, float* margin, float* cand6, uint8_t* conf, int16_t* q4,
+                     const Scratch& sp, const uint16_t* rf, const uint16_t* sc, bool store_vol,
+                     FrameDesc* fd) {
   std::memset(fd, 0, sizeof(*fd));
   const int D = c.dmax - c.dmin + 1;
   fd->L = L; fd->R = R;
@@ -175,7 +184,7 @@ int Ctx::frame_desc(const de_cfg& c, int W, int H, const uint8_t* L, const uint8
   fd->rf = rf; fd->sc = sc;
   fd->W = W; fd->H = H; fd->D = D; fd->G = (D + kWave - 1) / kWave; fd->Dpad = fd->G * kWave;
   fd->dmin = c.dmin; fd->hw = c.census_hw; fd->hh = c.census_hh;
-  fd->sc_shift = shift; fd->ad_T = std::min(255, std::max(1, c.ad_trunc));
+  fd->sc_shift = score_shift(c.ad_trunc); fd->ad_T = std::min(255, std::max(1, c.ad_trunc));
   // the exact reverse match reads the final volume, so the column pass stores it
   fd->lrc = (c.lrc && cand6) ? 1 : 0;
   fd->store_vol = (store_vol || fd->lrc) ? 1 : 0;
@@ -183,7 +192,6 @@ int Ctx::frame_desc(const de_cfg& c, int W, int H, const uint8_t* L, const uint8
   fd->iters = c.iters; fd->subpixel = c.subpixel; fd->fit_eq = c.fit_eq;
   fd->lambda = c.lambda; fd->gamma = c.gamma; fd->damping = c.damping;
   fd->min_margin = c.min_margin; fd->min_ratio = c.min_ratio;
-  return DE_OK;
 }
 
 size_t scratch_bytes(int W, int H, int D) {
@@ -472,6 +480,8 @@ int de_ctx_create(int device, de_ctx** out) {
   c->c.device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
+  // set-up of de_dense_run_dev_noalloc's scratch on this device (once per device)
+  if (int rc = noalloc_init_device(device)) { delete c; return rc; }
   *out = c;
   return DE_OK;
 }

@@ -44,6 +44,8 @@ struct Scratch {
   int16_t* rdisp;
 };
 size_t scratch_bytes(int W, int H, int D);
+void build_rf_table(float sigma_s, float sigma_r, uint16_t* tbl);
+int score_shift(int ad_trunc);
 Scratch carve_scratch(void* base, int W, int H, int D);
 
 struct Ctx {
@@ -113,6 +115,15 @@ struct Ctx {
   int dense_dev(const uint8_t* dL, const uint8_t* dR, int W, int H, const de_cfg& c,
                 float* disp, float* margin, float* cand6, hipStream_t s, int stage_mask);
 };
+
+void fill_frame_desc(const de_cfg& c, int W, int H, const uint8_t* L, const uint8_t* R,
+                     float* disp, float* margin, float* cand6, uint8_t* conf, int16_t* q4,
+                     const Scratch& sp, const uint16_t* rf, const uint16_t* sc, bool store_vol,
+                     FrameDesc* fd);
+
+// Device address of the allocation-free entry point's scratch array on the
+// current device, looked up once per device (de_noalloc.hip).
+int noalloc_init_device(int device);
 
 // kernel launchers
 size_t row_lds_bytes(int W, int nbits, int sc_shift);

@@ -314,6 +314,17 @@ class Context:
                                      C.c_void_p(d_cand6) if d_cand6 else None,
                                      C.c_void_p(stream) if stream else None))
 
+    def run_dense_dev_noalloc(self, dL: int, dR: int, W: int, H: int, cfg: Cfg, d_disp: int,
+                              d_margin: int, d_cand6: int = 0, stream: int = 0):
+        """run_dense_dev with no device allocation (the library's static scratch;
+        calls that overlap on one device race)."""
+        c = cfg.to_c()
+        check(lib().de_dense_run_dev_noalloc(self.h, C.c_void_p(dL), C.c_void_p(dR), W, H,
+                                             C.byref(c), C.c_void_p(d_disp),
+                                             C.c_void_p(d_margin),
+                                             C.c_void_p(d_cand6) if d_cand6 else None,
+                                             C.c_void_p(stream) if stream else None))
+
     def census_dev(self, d_img: int, W: int, H: int, hw: int, hh: int, d_out: int,
                    stream: int = 0):
         check(lib().de_census_dev(self.h, C.c_void_p(d_img), W, H, hw, hh, C.c_void_p(d_out),

@@ -102,6 +102,22 @@ int de_dense_run_dev(de_ctx* ctx, const uint8_t* d_L, const uint8_t* d_R, int W,
                      int H, const de_cfg* cfg, float* d_disp, float* d_margin,
                      float* d_cand6, void* stream);
 
+/* de_dense_run_dev without any device allocation. No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The frame's scratch,
+ * tables and launch descriptor live in one static device array of 256 MiB per
+ * device (held from library load to process end; its address is looked up
+ * when a context is created on the device), filled by kernels from their
+ * arguments: a call issues kernel launches only. Frames whose scratch exceeds
+ * de_dense_noalloc_limit() bytes (848x480 at 128 disparities fits) are refused
+ * with DE_ERR_ARG. Every call on a device shares the array: calls that overlap
+ * on one device (other host threads, unordered streams) race. Results equal
+ * de_dense_run_dev's. */
+int de_dense_run_dev_noalloc(de_ctx* ctx, const uint8_t* d_L, const uint8_t* d_R, int W, int H,
+                             const de_cfg* cfg, float* d_disp, float* d_margin, float* d_cand6,
+                             void* stream);
+size_t de_dense_noalloc_limit(void);
+
 /* The FP32 full-volume path (`de_dense --no-blockwise/--dump-vol`): the float
  * score and recursive filter (core/tools/de_dense.cpp:379-396, 510-527,
  * 1305-1318), volume [y][x][k] of W*H*(dmax-dmin+1) floats, -1e30 outside each
