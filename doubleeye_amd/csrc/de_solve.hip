@@ -425,6 +425,16 @@ size_t solve_lds_bytes(int W) {
          r(4 * size_t(kSolveThreads));
 }
 
+// Each instantiation may ask for up to 160 KiB of dynamic LDS (36 B a pixel at
+// rows of 4096); the attribute is set once per instantiation.
+template <int KP, int NT>
+static hipError_t solve_lds_attr() {
+  static const hipError_t attr = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(&k_solve<KP, NT>), hipFuncAttributeMaxDynamicSharedMemorySize,
+      160 * 1024);
+  return attr;
+}
+
 // maxW: widest row of the launch (at most kMaxW).
 hipError_t launch_solve(const BatchView& b, int items, size_t lds, int maxW, hipStream_t s) {
   // Rows of 513..1024 pixels: 1024 threads of one pixel each (at 848x480x128
@@ -432,13 +442,23 @@ hipError_t launch_solve(const BatchView& b, int items, size_t lds, int maxW, hip
   // against 0.139 ms a frame, two in flight 0.157 against 0.153). Otherwise
   // 512 threads of 1, 4 or 8 pixels.
   const int kp = (maxW + kSolveThreads - 1) / kSolveThreads;
-  if (kp <= 1) hipLaunchKernelGGL(k_solve<1>, dim3(items), dim3(kSolveThreads), lds, s, b);
-  else if (kp == 2)
+  hipError_t e = hipSuccess;
+  if (kp <= 1) {
+    if ((e = solve_lds_attr<1, kSolveThreads>()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_solve<1>, dim3(items), dim3(kSolveThreads), lds, s, b);
+  } else if (kp == 2) {
+    if ((e = solve_lds_attr<1, 2 * kSolveThreads>()) != hipSuccess) return e;
     hipLaunchKernelGGL((k_solve<1, 2 * kSolveThreads>), dim3(items), dim3(2 * kSolveThreads), lds,
                        s, b);
-  else if (kp <= 4) hipLaunchKernelGGL(k_solve<4>, dim3(items), dim3(kSolveThreads), lds, s, b);
-  else if (kp <= 8) hipLaunchKernelGGL(k_solve<8>, dim3(items), dim3(kSolveThreads), lds, s, b);
-  else return hipErrorInvalidValue;
+  } else if (kp <= 4) {
+    if ((e = solve_lds_attr<4, kSolveThreads>()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_solve<4>, dim3(items), dim3(kSolveThreads), lds, s, b);
+  } else if (kp <= 8) {
+    if ((e = solve_lds_attr<8, kSolveThreads>()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_solve<8>, dim3(items), dim3(kSolveThreads), lds, s, b);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
