@@ -258,6 +258,8 @@ int Ctx::prepare(const FrameDesc* fds_in, int n, Prepared* P) const {
       return fail(DE_ERR_ARG, "a batch must share one census window");
     const int nbits = (2 * fds[i].hw + 1) * (2 * fds[i].hh + 1) - 1;
     P->lds_row = std::max(P->lds_row, row_lds_bytes(fds[i].W, nbits, fds[i].sc_shift));
+    if (row_pass_min_lds(fds[i].W, fds[i].ad_T, fds[i].G) > 160 * 1024)
+      return fail(DE_ERR_ARG, "row of %d pixels too wide for the row pass's LDS", fds[i].W);
     P->lds_solve = std::max(P->lds_solve, solve_lds_bytes(fds[i].W));
     P->max_w = std::max(P->max_w, fds[i].W);
     if (size_t(fds[i].W) * fds[i].H * fds[i].D > size_t(fds[big].W) * fds[big].H * fds[big].D)
@@ -267,7 +269,12 @@ int Ctx::prepare(const FrameDesc* fds_in, int n, Prepared* P) const {
     return fail(DE_ERR_ARG, "row too wide for LDS (row %zu, solve %zu bytes)", P->lds_row,
                 P->lds_solve);
   // the variants of the largest frame's shape bucket drive the launch
-  P->rowv = pick_row_variant(fds[big].W, fds[big].H, fds[big].G);
+  // untuned row pass: the register-resident kernel when every row wave of the
+  // launch fits one per SIMD, the streaming one (two waves a SIMD) otherwise
+  size_t row_waves = 0;
+  for (int i = 0; i < n; ++i) row_waves += size_t(fds[i].H) * fds[i].G;
+  P->rowv = pick_row_variant(fds[big].W, fds[big].H, fds[big].G,
+                             row_waves <= size_t(device_simds()) ? 1 : 2);
   P->colv = pick_col_variant(fds[big].W, fds[big].H, fds[big].G);
   return DE_OK;
 }
@@ -296,8 +303,10 @@ int Ctx::launch_stages(const Prepared& P, const BatchView& bv, hipStream_t s, in
     return hipSuccess;
   };
   if ((stage_mask & 2) &&
-      (e = per_g(kStageRow, [&](int i, int, int base, int cnt) {
-         return launch_row(bv, base, cnt, fds[i].G, P.lds_row, P.rowv, s);
+      (e = per_g(kStageRow, [&](int i, int j, int base, int cnt) {
+         int maxw = 0;
+         for (int q = i; q < j; ++q) maxw = std::max(maxw, fds[q].W);
+         return launch_row(bv, base, cnt, fds[i].G, maxw, fds[i].ad_T, P.lds_row, P.rowv, s);
        })) != hipSuccess)
     return hip_fail(e, "k_row");
   if (ev) DE_HIP(hipEventRecord(ev->e[2], s));
@@ -365,11 +374,11 @@ int Ctx::dense_dev(const uint8_t* dL, const uint8_t* dR, int W, int H, const de_
 }
 
 // Defaults until the autotuner has measured a shape: 16 rows in flight in the
-// column pass, 32 columns in the row pass's backward scan.
-int Ctx::pick_row_variant(int W, int H, int G) const {
+// column pass; the row pass's fallback is chosen by the launch's size.
+int Ctx::pick_row_variant(int W, int H, int G, int fallback) const {
   if (force_row_variant >= 0) return force_row_variant;
   const auto it = tuned.find(std::make_tuple(W, H, G));
-  return it != tuned.end() ? it->second.first : 2;
+  return it != tuned.end() ? it->second.first : fallback;
 }
 int Ctx::pick_col_variant(int W, int H, int G) const {
   if (force_col_variant >= 0) return force_col_variant;

@@ -74,7 +74,7 @@ struct Ctx {
   // launch variants (prefetch depths) for the row and column passes; -1 = the
   // tuned or default choice for the shape
   int force_row_variant = -1, force_col_variant = -1;
-  int pick_row_variant(int W, int H, int G) const;
+  int pick_row_variant(int W, int H, int G, int fallback) const;
   int pick_col_variant(int W, int H, int G) const;
   // autotuned variants per shape bucket (W, H, G)
   std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;
@@ -129,8 +129,10 @@ int noalloc_init_device(int device);
 size_t row_lds_bytes(int W, int nbits, int sc_shift);
 size_t solve_lds_bytes(int W);
 hipError_t launch_prep(const BatchView& b, int items, int hw, int hh, hipStream_t s);
-hipError_t launch_row(const BatchView& b, int base, int items, int G, size_t lds, int variant,
-                      hipStream_t s);
+hipError_t launch_row(const BatchView& b, int base, int items, int G, int maxW, int T,
+                      size_t lds_wave, int variant, hipStream_t s);
+size_t row_pass_min_lds(int W, int T, int G);
+int device_simds();   // SIMDs of the current device (4 per CU)
 hipError_t launch_col(const BatchView& b, int base, int items, int G, bool store, int variant,
                       int min_h, hipStream_t s);
 hipError_t launch_solve(const BatchView& b, int items, size_t lds, int maxW, hipStream_t s);

@@ -20,15 +20,15 @@ extern const int kColVariants[kNumColVariants];
 constexpr int kVolPadRows = 64;
 constexpr int kAytPad = 64;       // words of padding around the transposed coefficients
 
-constexpr int kPrepTileW = 64;   // census/coefficient tile: 64 x 4 pixels
-constexpr int kPrepTileH = 4;
+constexpr int kPrepTileW = 64;   // census/coefficient tile: 64 x 8 pixels
+constexpr int kPrepTileH = 8;
 
 // Work items per frame for each stage.
 __host__ __device__ inline int stage_items(const FrameDesc& f, int stage) {
   switch (stage) {
     case kStagePrep:
       return ((f.W + kPrepTileW - 1) / kPrepTileW) * ((f.H + kPrepTileH - 1) / kPrepTileH);
-    case kStageRow: return f.H * f.G;     // one wave per (row, 64-disparity group)
+    case kStageRow: return f.H;           // one workgroup per row (a wave per 64-disparity group)
     case kStageCol: return f.W;           // one wave per column, all groups
     default: return f.H;                  // one workgroup per row
   }

@@ -50,7 +50,7 @@ struct FrameDesc {
   const uint8_t* L;      // left  image, W*H
   const uint8_t* R;      // right image, W*H
   uint32_t* lrec;        // left record per pixel, 4 words: census lo, census hi,
-                         //   2*L | (2*ax) << 16, 65536 - 2*ax (scalar loads)
+                         //   132*L, (2*ax) << 16 | min(65536 - 2*ax, 65535)
   uint64_t* cr;          // right census plane, W*H
   uint32_t* ayt;         // 2*ay, transposed [x][y] (scalar loads down a column)
   uint16_t* vol;         // aggregated volume v', [y][x][Dpad]

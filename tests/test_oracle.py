@@ -148,6 +148,16 @@ def test_filter_step_offset_identity():
     assert t2.max() < 2 ** 32 and acc.max() < 2 ** 32
     assert np.array_equal((t2 >> 16) - 16384, ref)
     assert ref.min() >= -16384 and ref.max() <= 16384
+    # the row pass's form: one filter word wz = 2a << 16 | min(65536 - 2a, 65535)
+    # and two 16 x 16 + 32 multiply-adds; at a = 0 the capped weight 65535 gives
+    # the same high half (65535 c' + 32768 >> 16 = c' for c' <= 32768)
+    wz = ((2 * a) << 16) | np.minimum(65536 - 2 * a, 65535)
+    acc16 = (wz & 0xFFFF) * cp + 32768
+    t2w = (wz >> 16) * fp + acc16
+    assert t2w.max() < 2 ** 32
+    assert np.array_equal((t2w >> 16) - 16384, ref)
+    zero = a == 0
+    assert zero.sum() >= 2 and np.array_equal((t2w[zero] >> 16) - 16384, c[zero])
 
 
 def _seq_top2(vals, kmax):
