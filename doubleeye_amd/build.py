@@ -27,6 +27,8 @@ COMMON = [
 ]
 SOURCES = {
     "de_dense.hip": [],
+    "de_row.hip": [],
+    "de_col.hip": [],
     "de_solve.hip": ["-ffp-contract=off"],
     "de_api.hip": ["-ffp-contract=off"],
     "de_sparse.hip": ["-ffp-contract=off"],

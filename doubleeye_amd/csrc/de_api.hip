@@ -274,7 +274,7 @@ int Ctx::prepare(const FrameDesc* fds_in, int n, Prepared* P) const {
   size_t row_waves = 0;
   for (int i = 0; i < n; ++i) row_waves += size_t(fds[i].H) * fds[i].G;
   P->rowv = pick_row_variant(fds[big].W, fds[big].H, fds[big].G,
-                             row_waves <= size_t(device_simds()) ? 1 : 2);
+                             row_waves <= size_t(device_simds()) ? 0 : 2);
   P->colv = pick_col_variant(fds[big].W, fds[big].H, fds[big].G);
   return DE_OK;
 }
