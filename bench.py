@@ -38,10 +38,11 @@ CFG = dict(CFG7, census_hw=4, census_hh=3)                 # C2 as stated: censu
 REF_FLAGS = ["--dmax", "128", "--iters", "8", "--sigma-s", "20", "--sigma-r", "0.0392"]
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of each stage kernel at C2 (census 9x7), from rocprofv3
-# FETCH_SIZE and WRITE_SIZE runs on this workload scaled by a known-size copy and
-# fill; a quoted profile figure, not measured in this run (None: not profiled).
-TRAFFIC: dict = {}
-TRAFFIC_SOURCE = "not profiled for this build"
+# FETCH_SIZE (x2) and WRITE_SIZE (x1) runs on this workload, the factors read off
+# a 256 MiB device copy in the same runs: a quoted profile figure, not measured
+# in this run; the source file is named in the output line.
+TRAFFIC = {"prep": 15_535_071, "row": 125_931_507, "col": 190_547_754, "solve": 9_888_890}
+TRAFFIC_SOURCE = "quoted: profiles/dense_c2_census9x7_848x480x128_kernels.md"
 
 
 def algorithmic_bytes(w: int, h: int, d: int) -> dict:
