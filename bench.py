@@ -762,6 +762,122 @@ def bench_full(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_host(a):
+    """Host-to-host C2 (848x480, d 1..128, census 9x7, 8 iterations): the pair
+    in host memory, disparity and margin back in host memory, one frame a
+    call, against the resident call (pair and outputs in HBM, a stream wait
+    after each call). Host legs: pageable numpy arrays (fresh outputs each
+    call, and reused ones), de_host_alloc (pinned) arrays, and the DEIR -> DEDD
+    stream front end (Q4 disparity and confidence out) fed one pair a call and
+    a whole run of pairs in one call (pairs in flight)."""
+    import numpy as np
+
+    from doubleeye_amd import synth
+    from doubleeye_amd.dense import Cfg, Context, host_empty
+    from doubleeye_amd.stream import Stream, deir_packet, parse_dedd
+
+    ctx = Context(0)
+    L, R, _ = synth.slanted_pair(W, H, seed=2)
+    cfg = Cfg(**CFG)
+    reps = max(10, a.steps // 4)
+
+    def timed(fn):
+        for _ in range(3):
+            out = fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        return (time.perf_counter() - t0) * 1e3 / reps, out
+
+    dL, dR = ctx.to_device(L), ctx.to_device(R)
+    dd, dm = ctx.empty((H, W), np.float32), ctx.empty((H, W), np.float32)
+
+    def res():
+        ctx.run_dense_dev(dL.ptr, dR.ptr, W, H, cfg, dd.ptr, dm.ptr)
+        ctx.sync()
+    ms_res = timed(res)[0]
+    want = dd.numpy()
+    ms_fresh, o1 = timed(lambda: ctx.run_dense(L, R, cfg))
+    reuse = (np.empty((H, W), np.float32), np.empty((H, W), np.float32))
+    ms_reuse, o2 = timed(lambda: ctx.run_dense(L, R, cfg, out=reuse))
+    pL, pR = host_empty((H, W), np.uint8), host_empty((H, W), np.uint8)
+    pL[...] = L
+    pR[...] = R
+    pin = (host_empty((H, W), np.float32), host_empty((H, W), np.float32))
+    ms_pin, o3 = timed(lambda: ctx.run_dense(pL, pR, cfg, out=pin))
+    same = all(np.array_equal(o[0].view(np.uint32), want.view(np.uint32)) for o in (o1, o2, o3))
+
+    n_pairs = 24
+    one = [deir_packet(L, 1, f) + deir_packet(R, 0, f) for f in range(n_pairs)]
+    st = Stream(ctx, W, H, cfg)
+    st.feed(one[0])
+    t0 = time.perf_counter()
+    for f in range(n_pairs):
+        st.feed(one[f])
+    ms_stream_one = (time.perf_counter() - t0) * 1e3 / n_pairs
+    blob = b"".join(one)
+    st.feed(blob)
+    t0 = time.perf_counter()
+    outb = st.feed(blob)
+    ms_stream_run = (time.perf_counter() - t0) * 1e3 / n_pairs
+    pk = parse_dedd(outb)
+    stream_ok = len(pk) == n_pairs and all(np.array_equal(p["q4"], pk[0]["q4"]) for p in pk)
+    st.close()
+    # the native fd loop (de_stream_run_fd, what `python -m doubleeye_amd.stream
+    # --stream` runs) from a file to /dev/null; its set-up (three captured plans)
+    # timed on an empty input and taken off
+    import ctypes as C
+
+    from doubleeye_amd._lib import check, lib
+
+    def run_fd(path):
+        fi, fo = os.open(path, os.O_RDONLY), os.open(os.devnull, os.O_WRONLY)
+        pairs = C.c_long(0)
+        t0 = time.perf_counter()
+        check(lib().de_stream_run_fd(ctx.h, fi, fo, W, H, C.byref(cfg.to_c()), 1,
+                                     C.byref(pairs), None))
+        dt = time.perf_counter() - t0
+        os.close(fi)
+        os.close(fo)
+        return dt, pairs.value
+    with tempfile.TemporaryDirectory() as td:
+        empty, full = os.path.join(td, "empty"), os.path.join(td, "run")
+        open(empty, "wb").close()
+        with open(full, "wb") as f:
+            for _ in range(4):
+                f.write(blob)
+        run_fd(empty)
+        t_set = min(run_fd(empty)[0] for _ in range(2))
+        t_run, fd_pairs = run_fd(full)
+    ms_fd = (t_run - t_set) * 1e3 / max(1, fd_pairs)
+    for b in (dL, dR, dd, dm):
+        b.free()
+    ctx.close()
+    out = {
+        "metric": "dense 848x480 D=128 census 9x7 host-to-host ms per pair",
+        "value": round(ms_pin, 4), "unit": "ms", "higher_is_better": False,
+        "resident_ms": round(ms_res, 4),
+        "host_pageable_fresh_ms": round(ms_fresh, 4),
+        "host_pageable_reused_ms": round(ms_reuse, 4),
+        "host_pinned_ms": round(ms_pin, 4),
+        "dedd_stream_one_pair_a_call_ms": round(ms_stream_one, 4),
+        "dedd_stream_pipelined_ms": round(ms_stream_run, 4),
+        "dedd_fd_loop_ms": round(ms_fd, 4), "dedd_fd_loop_pairs": fd_pairs,
+        "over_resident": {"pageable_fresh": round(ms_fresh / ms_res, 2),
+                          "pageable_reused": round(ms_reuse / ms_res, 2),
+                          "pinned": round(ms_pin / ms_res, 2),
+                          "dedd_pipelined": round(ms_stream_run / ms_res, 2),
+                          "dedd_fd_loop": round(ms_fd / ms_res, 2)},
+        "outputs_equal_resident": bool(same), "dedd_packets_ok": bool(stream_ok),
+        "n_gpus": 1, "steps": reps, "warmup": 3, "ms_per_step": round(ms_pin, 4),
+        "scaling": "weak", "vs_baseline": None, "dtype": "i16 volume / f32 messages",
+        "data": "synthetic (seeded slanted-plane pair, seed 2)",
+        "config": {"workload": "dense MASDA 848x480, D=128 (d=1..128), census 9x7, "
+                               "sigma_s 20, sigma_r 10/255, 8 BP iterations; host buffers"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_c2f(a):
     """The prior as a mask: one 848x480 slanted pair at D = 128 (d 0..127),
     8 iterations, through the full sweep, `--prior` (a full-resolution prior
@@ -855,11 +971,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--workload", default="c2",
                     choices=["c2", "c3", "c4", "c5", "sparse", "dense_masda", "detect",
-                             "pipe", "c2f", "full"],
+                             "pipe", "c2f", "full", "host"],
                     help="c2: one 848x480x128 pair (default, the headline); c4: 64-pair "
                          "batch; c3: keypoint-graph BP; c5: serving loop; sparse: keypoint "
                          "match_masda; detect: keypoint detector; pipe: de_pipe's pair; "
-                         "c2f: the prior mask; full: the FP32 full-volume path (--topk 0)")
+                         "c2f: the prior mask; full: the FP32 full-volume path (--topk 0); "
+                         "host: C2 from host buffers to host buffers")
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
     ap.add_argument("--frames-in-flight", type=int, default=4, choices=[1, 2, 3, 4],
                     help="c2: frames in flight, each on its own HIP stream (1: one at a time; "
@@ -908,6 +1025,9 @@ def main():
         return
     if a.workload == "full":
         bench_full(a)
+        return
+    if a.workload == "host":
+        bench_host(a)
         return
 
     import numpy as np

@@ -148,6 +148,8 @@ SIGNATURES = {
     "de_census": (I, [P, P, I, I, I, I, P]),
     "de_census_dev": (I, [P, P, I, I, I, I, P, P]),
     "de_dense_run": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, P, P]),
+    "de_host_alloc": (I, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "de_host_free": (I, [P]),
     "de_dense_run_dev": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, P, P, P]),
     "de_dense_run_dev_noalloc": (I, [P, P, P, I, I, C.POINTER(DeCfgC), P, P, P, P]),
     "de_dense_noalloc_limit": (C.c_size_t, []),

@@ -37,6 +37,7 @@ SOURCES = {
     "de_edges.hip": ["-ffp-contract=off"],
     "de_fp32.hip": ["-ffp-contract=off"],
     "de_stream.hip": [],
+    "de_hostio.hip": [],
     "de_detect.hip": ["-ffp-contract=off"],
     "de_prior.hip": [],
     "de_noalloc.hip": [],

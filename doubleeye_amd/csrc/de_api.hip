@@ -423,6 +423,7 @@ Ctx::~Ctx() {
                     &kb_scratch, &kb_io, &ed_scratch, &ed_io, &f32_scratch, &f32_io,
                     &det_scratch, &det_io, &pipe_io, &pr_buf, &pr_io})
     b->release();
+  hio.release();
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -614,7 +615,8 @@ int de_dense_run_dev(de_ctx* c, const uint8_t* dL, const uint8_t* dR, int W, int
   return c->c.dense_dev(dL, dR, W, H, *cfg, d_disp, d_margin, d_cand6, s, 0xF);
 }
 
-// Host-pointer form: stage the pair in the context's io buffers, run, copy back.
+// Host-pointer form: the pair goes up through the context's pinned staging,
+// the run is resident, the outputs come back in chunks (HostIo).
 static int dense_host(de_ctx* c, const uint8_t* L, const uint8_t* R, int W, int H,
                       const de_cfg* cfg, float* disp, float* margin, float* cand6,
                       int stage_mask) {
@@ -630,14 +632,13 @@ static int dense_host(de_ctx* c, const uint8_t* L, const uint8_t* R, int W, int 
   float* dd = static_cast<float*>(x.io_out.ptr);
   float* dm = dd + WH;
   float* dc = cand6 ? dm + WH : nullptr;
-  DE_HIP(hipMemcpyAsync(dL, L, WH, hipMemcpyHostToDevice, x.stream));
-  DE_HIP(hipMemcpyAsync(dR, R, WH, hipMemcpyHostToDevice, x.stream));
+  const Xfer in[2] = {{const_cast<uint8_t*>(L), dL, WH}, {const_cast<uint8_t*>(R), dR, WH}};
+  if (int rc = x.hio.put(in, 2, x.stream)) return rc;
   if (int rc = x.dense_dev(dL, dR, W, H, *cfg, dd, dm, dc, x.stream, stage_mask)) return rc;
-  if (disp) DE_HIP(hipMemcpyAsync(disp, dd, WH * 4, hipMemcpyDeviceToHost, x.stream));
-  if (margin) DE_HIP(hipMemcpyAsync(margin, dm, WH * 4, hipMemcpyDeviceToHost, x.stream));
-  if (cand6) DE_HIP(hipMemcpyAsync(cand6, dc, WH * 24, hipMemcpyDeviceToHost, x.stream));
-  DE_HIP(hipStreamSynchronize(x.stream));
-  return DE_OK;
+  const Xfer out[3] = {{disp, dd, disp ? WH * 4 : 0},
+                       {margin, dm, margin ? WH * 4 : 0},
+                       {cand6, dc, cand6 ? WH * 24 : 0}};
+  return x.hio.get(out, 3, x.stream);
 }
 
 int de_dense_run(de_ctx* c, const uint8_t* L, const uint8_t* R, int W, int H,

@@ -33,6 +33,42 @@ struct DevBuf {
   void release();
 };
 
+// Pinned host memory (hipHostMalloc), grown on demand.
+struct PinBuf {
+  void* ptr = nullptr;
+  size_t size = 0;
+  int ensure(size_t bytes);
+  void release();
+};
+
+// True when [p, p + n) lies in one allocation made by de_host_alloc: the
+// device copies from and to it directly.
+bool host_pinned(const void* p, size_t n);
+
+// One plane of a host-pointer entry point: `bytes` between a host and a device
+// address.
+struct Xfer {
+  void* host;
+  void* dev;
+  size_t bytes;
+};
+
+// Host <-> device transfers of the host-pointer entry points. A pageable
+// source is copied into pinned staging and its DMA queued at once, so the next
+// plane's host copy overlaps it; a pageable destination is fetched in chunks of
+// kChunk bytes, each copied out to the caller as soon as its DMA lands. Planes
+// in de_host_alloc memory move by DMA alone. get() returns when every plane is
+// in host memory.
+struct HostIo {
+  static constexpr size_t kChunk = size_t(1) << 20;
+  PinBuf in, out;
+  hipEvent_t in_free = nullptr;       // the last put's DMAs are done with `in`
+  std::vector<hipEvent_t> ev;         // one per chunk of a get
+  int put(const Xfer* x, int n, hipStream_t s);
+  int get(const Xfer* x, int n, hipStream_t s);
+  void release();
+};
+
 // One frame's device scratch.
 struct Scratch {
   uint32_t* lrec;
@@ -68,6 +104,7 @@ struct Ctx {
   DevBuf det_scratch, det_io; // keypoint detector: scratch, staged images and keypoints
   DevBuf pipe_io;             // detect -> match pipeline: descriptors and matches
   DevBuf pr_buf, pr_io;       // prior-masked sweep: mask planes, prior / coarse level
+  HostIo hio;                 // pinned staging of the host-pointer entry points
   Slot slots[kSlots];
   int next_slot = 0;
   bool store_vol = false;

@@ -210,7 +210,7 @@ static int c2f_dev(Ctx& x, const uint8_t* dL, const uint8_t* dR, int W, int H, d
   return prior_dev(x, dL, dR, W, H, c, prior, band, disp, margin, cand6, s);
 }
 
-// Stage a host pair (and optionally a host prior) in the context's io buffers.
+// Stage a host pair in the context's io buffers (through its pinned staging).
 static int stage_pair(Ctx& x, const uint8_t* L, const uint8_t* R, int W, int H, bool cand6,
                       uint8_t** dL, uint8_t** dR, float** dd, float** dm, float** dc) {
   const size_t WH = size_t(W) * H;
@@ -221,19 +221,20 @@ static int stage_pair(Ctx& x, const uint8_t* L, const uint8_t* R, int W, int H, 
   *dd = static_cast<float*>(x.io_out.ptr);
   *dm = *dd + WH;
   *dc = cand6 ? *dm + WH : nullptr;
-  DE_HIP(hipMemcpyAsync(*dL, L, WH, hipMemcpyHostToDevice, x.stream));
-  DE_HIP(hipMemcpyAsync(*dR, R, WH, hipMemcpyHostToDevice, x.stream));
-  return DE_OK;
+  const Xfer in[2] = {{const_cast<uint8_t*>(L), *dL, WH}, {const_cast<uint8_t*>(R), *dR, WH}};
+  return x.hio.put(in, 2, x.stream);
 }
 
+// Fetch the output planes (and the coarse level, when asked) and wait.
 static int fetch_outputs(Ctx& x, int W, int H, float* dd, float* dm, float* dc, float* disp,
-                         float* margin, float* cand6) {
+                         float* margin, float* cand6, float* coarse = nullptr,
+                         const float* dco = nullptr, size_t n_coarse = 0) {
   const size_t WH = size_t(W) * H;
-  if (disp) DE_HIP(hipMemcpyAsync(disp, dd, WH * 4, hipMemcpyDeviceToHost, x.stream));
-  if (margin) DE_HIP(hipMemcpyAsync(margin, dm, WH * 4, hipMemcpyDeviceToHost, x.stream));
-  if (cand6) DE_HIP(hipMemcpyAsync(cand6, dc, WH * 24, hipMemcpyDeviceToHost, x.stream));
-  DE_HIP(hipStreamSynchronize(x.stream));
-  return DE_OK;
+  const Xfer out[4] = {{disp, dd, disp ? WH * 4 : 0},
+                       {margin, dm, margin ? WH * 4 : 0},
+                       {cand6, dc, cand6 ? WH * 24 : 0},
+                       {coarse, const_cast<float*>(dco), coarse && dco ? n_coarse * 4 : 0}};
+  return x.hio.get(out, 4, x.stream);
 }
 
 }  // namespace de
@@ -264,10 +265,12 @@ int de_dense_run_prior(de_ctx* c, const uint8_t* L, const uint8_t* R, int W, int
   if (int rc = x.pr_io.ensure(a256(WH * 4) + a256(half * 4))) return rc;
   float* dprior = static_cast<float*>(x.pr_io.ptr);
   if (n_prior == WH) {
-    DE_HIP(hipMemcpyAsync(dprior, prior, WH * 4, hipMemcpyHostToDevice, x.stream));
+    const Xfer xp = {const_cast<float*>(prior), dprior, WH * 4};
+    if (int rc = x.hio.put(&xp, 1, x.stream)) return rc;
   } else {
     float* dhalf = dprior + a256(WH * 4) / 4;
-    DE_HIP(hipMemcpyAsync(dhalf, prior, half * 4, hipMemcpyHostToDevice, x.stream));
+    const Xfer xp = {const_cast<float*>(prior), dhalf, half * 4};
+    if (int rc = x.hio.put(&xp, 1, x.stream)) return rc;
     hipLaunchKernelGGL(k_upsample, dim3(unsigned((WH + 255) / 256)), dim3(256), 0, x.stream,
                        dhalf, W, H, dprior);
     DE_HIP(hipGetLastError());
@@ -292,9 +295,8 @@ int de_dense_run_c2f(de_ctx* c, const uint8_t* L, const uint8_t* R, int W, int H
   const float* dco = nullptr;
   int used = 0;
   if (int rc = c2f_dev(x, dL, dR, W, H, *cfg, band, dd, dm, dc, &dco, &used, x.stream)) return rc;
-  if (coarse) DE_HIP(hipMemcpyAsync(coarse, dco, n2 * 4, hipMemcpyDeviceToHost, x.stream));
   if (used_mask) *used_mask = used;
-  return fetch_outputs(x, W, H, dd, dm, dc, disp, margin, cand6);
+  return fetch_outputs(x, W, H, dd, dm, dc, disp, margin, cand6, coarse, dco, n2);
 }
 
 int de_dense_run_c2f_dev(de_ctx* c, const uint8_t* d_L, const uint8_t* d_R, int W, int H,

@@ -10,7 +10,16 @@
 // frame numbers are dropped); a second image of the same stream replaces the
 // first; stream 1 is the left camera. Fields are little-endian, as the
 // reference writes them with memcpy on little-endian hosts.
+//
+// Pairs are pipelined: up to kInFlight pairs run at once, each in a slot with
+// its own device buffers, pinned staging, captured plan and HIP stream, so one
+// pair's upload and another's download overlap a third's kernels. Packets
+// leave in pairing order. de_stream_feed returns every packet of the pairs
+// its bytes completed; de_stream_run_fd keeps pairs in flight while more
+// input is already waiting and writes them out before it blocks on a read.
 #include <hip/hip_runtime.h>
+#include <poll.h>
+#include <sys/uio.h>
 #include <unistd.h>
 
 #include <cstring>
@@ -26,10 +35,67 @@ namespace {
 constexpr size_t kHdrIn = 14, kHdrOut = 16;
 constexpr uint32_t kDeddConf = 1u;
 constexpr size_t kMaxPending = 8;
+constexpr int kInFlight = 3;
 
 struct Pending {
   std::vector<uint8_t> img;
   int stream;
+};
+
+size_t al256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Where finished DEDD packets go: the caller's buffer of de_stream_feed (a
+// packet that does not fit is dropped and flagged), or a file descriptor the
+// fd loop writes to straight from the pinned buffers.
+struct Sink {
+  uint8_t* p = nullptr;
+  size_t cap = 0, len = 0;
+  int fd = -1;
+  bool overflow = false;
+  int put(iovec* iov, int n) {
+    size_t bytes = 0;
+    for (int i = 0; i < n; ++i) bytes += iov[i].iov_len;
+    if (fd < 0) {
+      if (bytes > cap - len) {
+        overflow = true;
+        return DE_OK;
+      }
+      for (int i = 0; i < n; ++i) {
+        std::memcpy(p + len, iov[i].iov_base, iov[i].iov_len);
+        len += iov[i].iov_len;
+      }
+      return DE_OK;
+    }
+    while (n > 0) {
+      const ssize_t w = ::writev(fd, iov, n);
+      if (w <= 0) return fail(DE_ERR_ARG, "stream: output closed");
+      size_t done = size_t(w);
+      while (n > 0 && done >= iov->iov_len) {
+        done -= iov->iov_len;
+        ++iov;
+        --n;
+      }
+      if (n > 0) {
+        iov->iov_base = static_cast<uint8_t*>(iov->iov_base) + done;
+        iov->iov_len -= done;
+      }
+    }
+    return DE_OK;
+  }
+};
+
+// One pair in flight: device L | R | disp | margin | Q4 | conf, pinned
+// L | R | Q4 | conf, the pair's captured plan and its stream.
+struct Slot {
+  void* dev = nullptr;
+  void* pin = nullptr;
+  de_plan* plan = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t done = nullptr;
+  uint32_t frame = 0;
+  uint8_t *dL = nullptr, *dR = nullptr, *dconf = nullptr;
+  int16_t* dq4 = nullptr;
+  uint8_t *hL = nullptr, *hR = nullptr, *hq4 = nullptr, *hconf = nullptr;
 };
 
 }  // namespace
@@ -39,18 +105,12 @@ struct de_stream {
   int W = 0, H = 0;
   de_cfg cfg;
   bool conf = true;
-  std::vector<uint8_t> buf;                 // bytes not parsed yet
+  std::vector<uint8_t> buf;                 // bytes of an incomplete packet
   std::map<uint32_t, Pending> pending;
   long pairs = 0, dropped = 0;
-  void* dev = nullptr;
-  uint8_t* dL = nullptr;
-  uint8_t* dR = nullptr;
-  float* ddisp = nullptr;
-  float* dmargin = nullptr;
-  int16_t* dq4 = nullptr;
-  uint8_t* dconf = nullptr;
-  std::vector<int16_t> q4;
-  std::vector<uint8_t> cf;
+  Slot slot[kInFlight];
+  int head = 0;                             // the next slot to launch into
+  int busy = 0;                             // slots in flight, the oldest at head - busy
 };
 
 namespace {
@@ -60,96 +120,122 @@ size_t packet_out_bytes(int W, int H, bool conf) {
   return kHdrOut + WH + WH * 2 + (conf ? WH : 0);
 }
 
-// Run one pair and append its DEDD packet.
-int process_pair(de_stream* s, const std::vector<uint8_t>& lft, const std::vector<uint8_t>& rgt,
-                 uint32_t frame, std::vector<uint8_t>* out) {
+// Wait for the oldest pair in flight and append its DEDD packet.
+int finish_oldest(de_stream* s, Sink& out) {
+  Slot& sl = s->slot[(s->head - s->busy + kInFlight) % kInFlight];
+  DE_HIP(hipEventSynchronize(sl.done));
+  --s->busy;
+  ++s->pairs;
   const size_t WH = size_t(s->W) * s->H;
-  {
-    std::lock_guard<std::mutex> lk(s->ctx->c.mu);
-    DE_HIP(hipSetDevice(s->ctx->c.device));
-    DE_HIP(hipMemcpyAsync(s->dL, lft.data(), WH, hipMemcpyHostToDevice, s->ctx->c.stream));
-    DE_HIP(hipMemcpyAsync(s->dR, rgt.data(), WH, hipMemcpyHostToDevice, s->ctx->c.stream));
-  }
-  de_pair p;
-  std::memset(&p, 0, sizeof(p));
-  p.L = s->dL;
-  p.R = s->dR;
-  p.W = s->W;
-  p.H = s->H;
-  p.disp = s->ddisp;
-  p.margin = s->dmargin;
-  p.conf = s->conf ? s->dconf : nullptr;
-  p.q4 = s->dq4;
-  if (int rc = de_dense_batch_dev(s->ctx, &p, &s->cfg, 1, nullptr)) return rc;
-  {
-    std::lock_guard<std::mutex> lk(s->ctx->c.mu);
-    DE_HIP(hipMemcpyAsync(s->q4.data(), s->dq4, WH * 2, hipMemcpyDeviceToHost, s->ctx->c.stream));
-    if (s->conf)
-      DE_HIP(hipMemcpyAsync(s->cf.data(), s->dconf, WH, hipMemcpyDeviceToHost, s->ctx->c.stream));
-    DE_HIP(hipStreamSynchronize(s->ctx->c.stream));
-  }
-  unsigned char hdr[kHdrOut];
-  std::memcpy(hdr, "DEDD", 4);
+  uint8_t hdr[kHdrOut];
   const uint16_t pw = uint16_t(s->W), ph = uint16_t(s->H);
   const uint32_t flags = s->conf ? kDeddConf : 0u;
+  std::memcpy(hdr, "DEDD", 4);
   std::memcpy(hdr + 4, &pw, 2);
   std::memcpy(hdr + 6, &ph, 2);
-  std::memcpy(hdr + 8, &frame, 4);
+  std::memcpy(hdr + 8, &sl.frame, 4);
   std::memcpy(hdr + 12, &flags, 4);
-  out->insert(out->end(), hdr, hdr + kHdrOut);
-  out->insert(out->end(), lft.begin(), lft.end());
-  const uint8_t* q = reinterpret_cast<const uint8_t*>(s->q4.data());
-  out->insert(out->end(), q, q + WH * 2);
-  if (s->conf) out->insert(out->end(), s->cf.begin(), s->cf.end());
-  ++s->pairs;
+  iovec iov[4] = {{hdr, kHdrOut}, {sl.hL, WH}, {sl.hq4, WH * 2}, {sl.hconf, WH}};
+  return out.put(iov, s->conf ? 4 : 3);
+}
+
+int finish_all(de_stream* s, Sink& out) {
+  while (s->busy > 0)
+    if (int rc = finish_oldest(s, out)) return rc;
   return DE_OK;
 }
 
-// Parse every complete DEIR packet in the buffer; pairs produce DEDD packets.
-int drain(de_stream* s, std::vector<uint8_t>* out) {
+// Launch one pair in the next slot (first finishing the pair that held it).
+int launch_pair(de_stream* s, const uint8_t* lft, const uint8_t* rgt, uint32_t frame,
+                Sink& out) {
+  if (s->busy == kInFlight)
+    if (int rc = finish_oldest(s, out)) return rc;
+  Slot& sl = s->slot[s->head];
   const size_t WH = size_t(s->W) * s->H;
-  size_t pos = 0;
-  int rc = DE_OK;
-  while (s->buf.size() - pos >= kHdrIn) {
-    const uint8_t* h = s->buf.data() + pos;
-    if (std::memcmp(h, "DEIR", 4) != 0) {
-      rc = fail(DE_ERR_ARG, "stream: lost sync (bad magic)");
-      break;
-    }
-    uint16_t pw, ph;
-    uint32_t num;
-    std::memcpy(&pw, h + 4, 2);
-    std::memcpy(&ph, h + 6, 2);
-    const int st = h[8];
-    std::memcpy(&num, h + 10, 4);
-    if (int(pw) != s->W || int(ph) != s->H) {
-      rc = fail(DE_ERR_ARG, "stream: packet is %dx%d, the stream was opened for %dx%d",
-                int(pw), int(ph), s->W, s->H);
-      break;
-    }
-    if (s->buf.size() - pos < kHdrIn + WH) break;   // payload incomplete
-    std::vector<uint8_t> img(h + kHdrIn, h + kHdrIn + WH);
-    pos += kHdrIn + WH;
-    auto it = s->pending.find(num);
-    if (it == s->pending.end()) {
-      s->pending[num] = Pending{std::move(img), st};
-      while (s->pending.size() > kMaxPending) {
-        s->pending.erase(s->pending.begin());
-        ++s->dropped;
-      }
-      continue;
-    }
-    if (it->second.stream == st) {
-      it->second.img = std::move(img);
-      continue;
-    }
-    const std::vector<uint8_t>& lft = (st == 1) ? img : it->second.img;
-    const std::vector<uint8_t>& rgt = (st == 1) ? it->second.img : img;
-    rc = process_pair(s, lft, rgt, num, out);
+  DE_HIP(hipSetDevice(s->ctx->c.device));
+  std::memcpy(sl.hL, lft, WH);
+  DE_HIP(hipMemcpyAsync(sl.dL, sl.hL, WH, hipMemcpyHostToDevice, sl.st));
+  std::memcpy(sl.hR, rgt, WH);
+  DE_HIP(hipMemcpyAsync(sl.dR, sl.hR, WH, hipMemcpyHostToDevice, sl.st));
+  if (int rc = de_plan_launch(sl.plan, sl.st)) return rc;
+  DE_HIP(hipMemcpyAsync(sl.hq4, sl.dq4, WH * 2, hipMemcpyDeviceToHost, sl.st));
+  if (s->conf) DE_HIP(hipMemcpyAsync(sl.hconf, sl.dconf, WH, hipMemcpyDeviceToHost, sl.st));
+  DE_HIP(hipEventRecord(sl.done, sl.st));
+  sl.frame = frame;
+  s->head = (s->head + 1) % kInFlight;
+  ++s->busy;
+  return DE_OK;
+}
+
+// The --stream pairing rule: an image waits under its frame number until the
+// other camera's image of that frame arrives, and then the pair runs (stream
+// 1 is the left camera); a newer image from the same camera replaces the
+// waiting one; past kMaxPending waiting frames the lowest frame numbers are
+// dropped.
+int accept_image(de_stream* s, uint32_t num, int cam, const uint8_t* img, Sink& out) {
+  const size_t WH = size_t(s->W) * s->H;
+  const auto it = s->pending.find(num);
+  if (it != s->pending.end() && it->second.stream != cam) {
+    const uint8_t* waiting = it->second.img.data();
+    const int rc = cam == 1 ? launch_pair(s, img, waiting, num, out)
+                            : launch_pair(s, waiting, img, num, out);
     s->pending.erase(it);
-    if (rc != DE_OK) break;
+    return rc;
   }
-  s->buf.erase(s->buf.begin(), s->buf.begin() + long(pos));
+  Pending& w = s->pending[num];
+  w.img.assign(img, img + WH);
+  w.stream = cam;
+  for (; s->pending.size() > kMaxPending; ++s->dropped) s->pending.erase(s->pending.begin());
+  return DE_OK;
+}
+
+// Consume every complete DEIR packet of [p, p + n) (header: "DEIR", w u16,
+// h u16, stream u8, flags u8, frame u32; then w*h bytes); returns the bytes
+// consumed.
+size_t parse(de_stream* s, const uint8_t* p, size_t n, Sink& out, int* rc) {
+  const size_t whole = kHdrIn + size_t(s->W) * s->H;
+  size_t used = 0;
+  *rc = DE_OK;
+  while (n - used >= kHdrIn) {
+    const uint8_t* h = p + used;
+    if (std::memcmp(h, "DEIR", 4) != 0) {
+      *rc = fail(DE_ERR_ARG, "stream: lost sync (bad magic)");
+      break;
+    }
+    uint16_t w16, h16;
+    uint32_t num;
+    std::memcpy(&w16, h + 4, 2);
+    std::memcpy(&h16, h + 6, 2);
+    std::memcpy(&num, h + 10, 4);
+    if (w16 != s->W || h16 != s->H) {
+      *rc = fail(DE_ERR_ARG, "stream: packet is %dx%d, the stream was opened for %dx%d",
+                 int(w16), int(h16), s->W, s->H);
+      break;
+    }
+    if (n - used < whole) break;   // payload not all here yet
+    used += whole;
+    if ((*rc = accept_image(s, num, h[8], h + kHdrIn, out)) != DE_OK) break;
+  }
+  return used;
+}
+
+// Feed bytes: parse them in place (after the held bytes of an incomplete
+// packet, if any) and hold what is left of an incomplete packet. With
+// `finish`, also wait for every pair in flight.
+int drain(de_stream* s, const uint8_t* data, size_t n, Sink& out, bool finish) {
+  int rc = DE_OK;
+  if (s->buf.empty()) {
+    const size_t used = parse(s, data, n, out, &rc);
+    s->buf.assign(data + used, data + n);
+  } else {
+    s->buf.insert(s->buf.end(), data, data + n);
+    const size_t used = parse(s, s->buf.data(), s->buf.size(), out, &rc);
+    s->buf.erase(s->buf.begin(), s->buf.begin() + long(used));
+  }
+  if (finish) {
+    const int rc2 = finish_all(s, out);
+    if (rc == DE_OK) rc = rc2;
+  }
   return rc;
 }
 
@@ -159,6 +245,19 @@ extern "C" {
 
 size_t de_stream_packet_bytes(int W, int H, int want_conf) {
   return (W > 0 && H > 0) ? packet_out_bytes(W, H, want_conf != 0) : 0;
+}
+
+static void release_slots(de_stream* s) {
+  (void)hipSetDevice(s->ctx->c.device);
+  for (Slot& sl : s->slot) {
+    if (sl.st) (void)hipStreamSynchronize(sl.st);
+    if (sl.plan) de_plan_destroy(sl.plan);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.st) (void)hipStreamDestroy(sl.st);
+    if (sl.dev) (void)hipFree(sl.dev);
+    if (sl.pin) (void)hipHostFree(sl.pin);
+    sl = Slot();
+  }
 }
 
 int de_stream_create(de_ctx* ctx, int W, int H, const de_cfg* cfg, int want_conf,
@@ -175,59 +274,68 @@ int de_stream_create(de_ctx* ctx, int W, int H, const de_cfg* cfg, int want_conf
   s->cfg = *cfg;
   s->conf = want_conf != 0;
   const size_t WH = size_t(W) * H;
-  auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
-  const size_t bytes = al(WH) * 2 + al(WH * 4) * 2 + al(WH * 2) + al(WH);
-  {
-    std::lock_guard<std::mutex> lk(ctx->c.mu);
+  const size_t dev_bytes = al256(WH) * 2 + al256(WH * 4) * 2 + al256(WH * 2) + al256(WH);
+  const size_t pin_bytes = al256(WH) * 2 + al256(WH * 2) + al256(WH);
+  auto bail = [&](int rc) {
+    release_slots(s);
+    delete s;
+    return rc;
+  };
+  for (Slot& sl : s->slot) {
     hipError_t e = hipSetDevice(ctx->c.device);
-    if (e == hipSuccess) e = hipMalloc(&s->dev, bytes);
-    if (e != hipSuccess) {
-      delete s;
-      return hip_fail(e, "stream buffers");
-    }
+    if (e == hipSuccess) e = hipMalloc(&sl.dev, dev_bytes);
+    if (e == hipSuccess) e = hipHostMalloc(&sl.pin, pin_bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e != hipSuccess) return bail(hip_fail(e, "stream slot"));
+    uint8_t* p = static_cast<uint8_t*>(sl.dev);
+    sl.dL = p;
+    sl.dR = p + al256(WH);
+    float* ddisp = reinterpret_cast<float*>(p + 2 * al256(WH));
+    float* dmargin = reinterpret_cast<float*>(p + 2 * al256(WH) + al256(WH * 4));
+    sl.dq4 = reinterpret_cast<int16_t*>(p + 2 * al256(WH) + 2 * al256(WH * 4));
+    sl.dconf = p + 2 * al256(WH) + 2 * al256(WH * 4) + al256(WH * 2);
+    uint8_t* h = static_cast<uint8_t*>(sl.pin);
+    sl.hL = h;
+    sl.hR = h + al256(WH);
+    sl.hq4 = h + 2 * al256(WH);
+    sl.hconf = h + 2 * al256(WH) + al256(WH * 2);
+    de_pair pr;
+    std::memset(&pr, 0, sizeof(pr));
+    pr.L = sl.dL;
+    pr.R = sl.dR;
+    pr.W = W;
+    pr.H = H;
+    pr.disp = ddisp;
+    pr.margin = dmargin;
+    pr.conf = s->conf ? sl.dconf : nullptr;
+    pr.q4 = sl.dq4;
+    if (int rc = de_plan_create(ctx, &pr, cfg, 1, &sl.plan)) return bail(rc);
   }
-  uint8_t* p = static_cast<uint8_t*>(s->dev);
-  s->dL = p;
-  p += al(WH);
-  s->dR = p;
-  p += al(WH);
-  s->ddisp = reinterpret_cast<float*>(p);
-  p += al(WH * 4);
-  s->dmargin = reinterpret_cast<float*>(p);
-  p += al(WH * 4);
-  s->dq4 = reinterpret_cast<int16_t*>(p);
-  p += al(WH * 2);
-  s->dconf = p;
-  s->q4.resize(WH);
-  s->cf.resize(s->conf ? WH : 0);
   *out = s;
   return DE_OK;
 }
 
 void de_stream_destroy(de_stream* s) {
   if (!s) return;
-  {
-    std::lock_guard<std::mutex> lk(s->ctx->c.mu);
-    (void)hipSetDevice(s->ctx->c.device);
-    (void)hipStreamSynchronize(s->ctx->c.stream);
-    if (s->dev) (void)hipFree(s->dev);
-  }
+  release_slots(s);
   delete s;
 }
 
 int de_stream_feed(de_stream* s, const uint8_t* data, size_t n, uint8_t* out, size_t out_cap,
                    size_t* out_len) {
-  if (!s || (n && !data) || !out_len) return fail(DE_ERR_ARG, "de_stream_feed: null argument");
+  if (!s || (n && !data) || !out_len || (out_cap && !out))
+    return fail(DE_ERR_ARG, "de_stream_feed: null argument");
   *out_len = 0;
-  s->buf.insert(s->buf.end(), data, data + n);
-  std::vector<uint8_t> o;
-  const int rc = drain(s, &o);
-  if (o.size() > out_cap) {
-    return fail(DE_ERR_ARG, "de_stream_feed: %zu output bytes exceed capacity %zu (size it "
-                "with de_stream_packet_bytes per pair fed)", o.size(), out_cap);
-  }
-  if (!o.empty()) std::memcpy(out, o.data(), o.size());
-  *out_len = o.size();
+  Sink sk;
+  sk.p = out;
+  sk.cap = out_cap;
+  const int rc = drain(s, data, n, sk, true);
+  *out_len = sk.len;
+  if (rc == DE_OK && sk.overflow)
+    return fail(DE_ERR_ARG, "de_stream_feed: the output capacity %zu holds %zu bytes of packets; "
+                "the pairs past it were dropped (size it with de_stream_packet_bytes per pair "
+                "fed)", out_cap, sk.len);
   return rc;
 }
 
@@ -247,19 +355,17 @@ int de_stream_run_fd(de_ctx* ctx, int in_fd, int out_fd, int W, int H, const de_
   for (;;) {
     const ssize_t got = ::read(in_fd, in.data(), in.size());
     if (got <= 0) break;   // end of input (or a read error): stop like the reference
-    s->buf.insert(s->buf.end(), in.data(), in.data() + got);
-    std::vector<uint8_t> o;
-    rc = drain(s, &o);
-    size_t off = 0;
-    while (off < o.size()) {
-      const ssize_t w = ::write(out_fd, o.data() + off, o.size() - off);
-      if (w <= 0) {
-        rc = fail(DE_ERR_ARG, "stream: output closed");
-        break;
-      }
-      off += size_t(w);
-    }
-    if (rc != DE_OK) break;
+    // keep pairs in flight only while more input is already waiting
+    pollfd pf = {in_fd, POLLIN, 0};
+    const bool more = ::poll(&pf, 1, 0) > 0;
+    Sink sk;
+    sk.fd = out_fd;
+    if ((rc = drain(s, in.data(), size_t(got), sk, !more)) != DE_OK) break;
+  }
+  if (rc == DE_OK) {
+    Sink sk;
+    sk.fd = out_fd;
+    rc = finish_all(s, sk);
   }
   if (pairs) *pairs = s->pairs;
   if (dropped) *dropped = s->dropped;

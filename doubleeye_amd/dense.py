@@ -21,7 +21,25 @@ import numpy as np
 from ._lib import DeCfgC, DePairC, check, lib
 
 __all__ = ["Cfg", "Context", "DeviceArray", "Pair", "Plan", "run_dense", "census",
-           "default_context"]
+           "default_context", "host_empty"]
+
+
+def host_empty(shape, dtype=np.uint8) -> np.ndarray:
+    """An uninitialised numpy array in pinned host memory (de_host_alloc).
+
+    The host-pointer entry points move such arrays by DMA alone; ordinary
+    arrays go through the context's pinned staging. The memory is returned
+    with de_host_free when the last view of the array is gone."""
+    import weakref
+
+    dt = np.dtype(dtype)
+    n = max(1, int(np.prod(shape, dtype=np.int64)) * dt.itemsize)
+    p = C.c_void_p()
+    check(lib().de_host_alloc(n, C.byref(p)))
+    buf = (C.c_uint8 * n).from_address(p.value)
+    weakref.finalize(buf, lib().de_host_free, p.value)
+    return np.frombuffer(buf, np.uint8, int(np.prod(shape, dtype=np.int64)) * dt.itemsize) \
+        .view(dt).reshape(shape)
 
 
 @dataclass
@@ -193,16 +211,28 @@ class Context:
         return out
 
     def run_dense(self, L: np.ndarray, R: np.ndarray, cfg: Cfg | None = None,
-                  want_cand: bool = False):
+                  want_cand: bool = False, out=None):
+        """(disp, margin, cand6 or None). `out`: (disp, margin[, cand6]) arrays to
+        write into (float32, C-contiguous, H x W and 6 x H x W), e.g. from
+        host_empty, instead of new ones."""
         cfg = cfg or Cfg()
         L = np.ascontiguousarray(L, np.uint8)
         R = np.ascontiguousarray(R, np.uint8)
         if L.shape != R.shape or L.ndim != 2:
             raise ValueError(f"left {L.shape} and right {R.shape} must be equal 2-D images")
         H, W = L.shape
-        disp = np.empty((H, W), np.float32)
-        margin = np.empty((H, W), np.float32)
-        cand = np.empty((6, H, W), np.float32) if want_cand else None
+        if out is not None:
+            disp, margin = out[0], out[1]
+            cand = out[2] if want_cand else None
+            for a, shp in ((disp, (H, W)), (margin, (H, W)), (cand, (6, H, W))):
+                if a is not None and (a.dtype != np.float32 or a.shape != shp
+                                      or not a.flags.c_contiguous):
+                    raise ValueError(f"out array {a.dtype} {a.shape}: expected float32 {shp}, "
+                                     "C-contiguous")
+        else:
+            disp = np.empty((H, W), np.float32)
+            margin = np.empty((H, W), np.float32)
+            cand = np.empty((6, H, W), np.float32) if want_cand else None
         c = cfg.to_c()
         check(lib().de_dense_run(self.h, _p(L), _p(R), W, H, C.byref(c), _p(disp), _p(margin),
                                  _p(cand) if cand is not None else None))

@@ -61,20 +61,25 @@ class Stream:
 
     def __init__(self, ctx, W: int, H: int, cfg: Cfg, want_conf: bool = True):
         self.W, self.H, self.conf = W, H, want_conf
+        self._out = None
         h = C.c_void_p()
         check(lib().de_stream_create(ctx.h, W, H, C.byref(cfg.to_c()), int(want_conf),
                                      C.byref(h)))
         self.h = h.value
 
-    def feed(self, data: bytes) -> bytes:
+    def feed(self, data) -> bytes:
+        """Feed DEIR bytes; return the DEDD packets of every pair they completed."""
+        if not isinstance(data, bytes):
+            data = bytes(data)
         # every complete image can finish at most one pair
         pairs = len(data) // (self.W * self.H) + 2
         cap = pairs * lib().de_stream_packet_bytes(self.W, self.H, int(self.conf))
-        out = (C.c_uint8 * max(cap, 1))()
+        if self._out is None or self._out.size < cap:
+            self._out = np.empty(max(cap, 1), np.uint8)      # kept for later calls
         n = C.c_size_t(0)
-        src = (C.c_uint8 * max(len(data), 1)).from_buffer_copy(data or b"\0")
-        check(lib().de_stream_feed(self.h, src, len(data), out, cap, C.byref(n)))
-        return bytes(out[: n.value])
+        check(lib().de_stream_feed(self.h, data, len(data), self._out.ctypes.data, cap,
+                                   C.byref(n)))
+        return self._out[: n.value].tobytes()
 
     def stats(self):
         p, d = C.c_long(0), C.c_long(0)

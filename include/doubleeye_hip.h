@@ -98,6 +98,16 @@ int de_census_dev(de_ctx* ctx, const uint8_t* d_img, int W, int H, int hw, int h
  * (the row solver holds a row in LDS); wider images return DE_ERR_ARG. */
 int de_dense_run(de_ctx* ctx, const uint8_t* L, const uint8_t* R, int W, int H,
                  const de_cfg* cfg, float* disp, float* margin, float* cand6);
+
+/* Pinned host memory for the host-pointer entry points. The reference's
+ * run_dense takes ordinary heap buffers (core/tools/de_dense.cpp:738-741:
+ * Image8 pair, std::vector output planes, loaded at :1751); every host-pointer
+ * entry point accepts those, staging them through the context's own pinned
+ * buffers. Buffers from
+ * de_host_alloc are read and written by DMA directly, with no host copy.
+ * de_host_free takes only what de_host_alloc returned. */
+int de_host_alloc(size_t bytes, void** out);
+int de_host_free(void* p);
 int de_dense_run_dev(de_ctx* ctx, const uint8_t* d_L, const uint8_t* d_R, int W,
                      int H, const de_cfg* cfg, float* d_disp, float* d_margin,
                      float* d_cand6, void* stream);
@@ -554,12 +564,17 @@ size_t de_stream_packet_bytes(int W, int H, int want_conf);
 int de_stream_create(de_ctx* ctx, int W, int H, const de_cfg* cfg, int want_conf,
                      de_stream** out);
 void de_stream_destroy(de_stream* s);
-/* Feed any number of input bytes; complete DEDD packets are written to out
- * (capacity out_cap; de_stream_packet_bytes per pair completed). */
+/* Feed any number of input bytes; the DEDD packets of every pair they
+ * complete are written to out (capacity out_cap; de_stream_packet_bytes per
+ * pair completed), *out_len bytes. Up to three pairs run at once inside one
+ * call (own streams and plans); packets keep pairing order. Packets past
+ * out_cap are dropped and the call returns DE_ERR_ARG. */
 int de_stream_feed(de_stream* s, const uint8_t* data, size_t n, uint8_t* out, size_t out_cap,
                    size_t* out_len);
 int de_stream_stats(de_stream* s, long* pairs, long* dropped);
-/* The whole loop over file descriptors until end of input. */
+/* The whole loop over file descriptors until end of input. Pairs stay in
+ * flight while more input is already readable; packets are written before
+ * the loop blocks on a read, and at the end. */
 int de_stream_run_fd(de_ctx* ctx, int in_fd, int out_fd, int W, int H, const de_cfg* cfg,
                      int want_conf, long* pairs, long* dropped);
 

@@ -60,3 +60,14 @@ def test_deir_dedd_packet_helpers():
     (d,) = parse_dedd(pkt)
     assert d["frame"] == 77 and np.array_equal(d["left"], img)
     assert np.array_equal(d["q4"], q4) and np.array_equal(d["conf"], conf)
+
+
+def test_host_free_rejects_foreign_pointer():
+    """de_host_free takes only what de_host_alloc returned; the check is on the
+    host, before any HIP call."""
+    import numpy as np
+
+    from doubleeye_amd import _lib
+    buf = np.zeros(16, np.uint8)
+    assert _lib.lib().de_host_free(buf.ctypes.data) == -1      # DE_ERR_ARG
+    assert _lib.lib().de_host_free(None) == 0

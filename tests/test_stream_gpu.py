@@ -109,6 +109,20 @@ def test_feed_in_chunks(ctx, oracle):
     s.close()
 
 
+def test_feed_whole_stream_at_once(ctx):
+    """Every pair of the stream in one feed call: more pairs than slots in
+    flight, so slots are reused while earlier pairs are still on the GPU."""
+    from doubleeye_amd.dense import Cfg
+    from doubleeye_amd.stream import Stream, parse_dedd
+    data, want, dropped = packets_and_model()
+    s = Stream(ctx, W, H, Cfg(**KW))
+    out = s.feed(data)
+    note(f"[stream] one feed, {len(want)} pairs")
+    check_packets(ctx, parse_dedd(out), want)
+    assert s.stats() == (len(want), dropped)
+    s.close()
+
+
 def test_bad_magic_raises(ctx):
     from doubleeye_amd._lib import DeError
     from doubleeye_amd.dense import Cfg
