@@ -107,7 +107,11 @@ def bench_c4(a):
     ref = [t["disp"].numpy() for t in items]
     for t in items:
         ctx.fill_dev(t["disp"], 0)
+    if a.row_variant >= 0:              # the batch's row-pass variant (plain calls keep theirs)
+        ctx.set_variants(a.row_variant, -1)
     plan = ctx.plan([t["pair"] for t in items], [t["cfg"] for t in items])
+    ctx.set_variants(-1, -1)
+    parts = plan.parts
     plan.launch()
     ctx.sync()
     same = all(np.array_equal(t["disp"].numpy().view(np.uint32), r.view(np.uint32))
@@ -137,6 +141,7 @@ def bench_c4(a):
         "plain_ms_per_batch": round(ms_plain, 3),
         "plain_value": round(pxd / (ms_plain * 1e-3) / 1e6, 1),
         "batched_over_plain": round(ms_plain / ms_batch, 3),
+        "plan_parts": parts,
         "identical_to_plain": same,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "i16 volume / f32 messages",
@@ -985,8 +990,8 @@ def main():
                     help="c2: also run the 7x7 census beside the stated 9x7 (both, the "
                          "default), or 9x7 alone (profiling runs)")
     ap.add_argument("--row-variant", type=int, default=-1,
-                    help="c2: force the row pass's launch variant (0-3; -1: the library's "
-                         "choice)")
+                    help="c2, c4: force the row pass's launch variant (0-3; -1: the "
+                         "library's choice; c4: the batch's plan only)")
     ap.add_argument("--c5-seconds", type=float, default=20.0,
                     help="c2: seconds of the 16-stream serving leg reported beside the "
                          "headline (p50/p99 latency); 0 skips it")

@@ -162,6 +162,7 @@ SIGNATURES = {
     "de_plan_create": (I, [P, C.POINTER(DePairC), C.POINTER(DeCfgC), I, C.POINTER(P)]),
     "de_plan_launch": (I, [P, P]),
     "de_plan_destroy": (None, [P]),
+    "de_plan_parts": (I, [P]),
     "de_autotune": (I, [P, I, I, C.POINTER(DeCfgC), I, C.POINTER(I), C.POINTER(I)]),
     "de_match_default_cfg": (None, [C.POINTER(DeMatchCfgC)]),
     "de_match_masda": (I, [P, P, P, I, P, P, I, C.POINTER(DeMatchCfgC), P, P, P,

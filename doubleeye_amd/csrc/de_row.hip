@@ -50,9 +50,11 @@ namespace de {
 //
 // The forward scan is software-pipelined over 8-column blocks in three
 // stages (LDS loads of block n+2, score-table reads of block n+1, chain of
-// block n). The backward scan's final values are transposed through two
-// alternating 1 KB LDS tiles and leave as 16-byte stores into the [y][x][Dpad]
-// volume, lane l writing disparities 8(l%8)..8(l%8)+7 of column 8b + l/8.
+// block n). The backward scan's final values leave as one 2-byte store a lane
+// a column (global_store_short_d16_hi of the chain word), a wave's store
+// covering the 128 contiguous bytes of its disparity group in the [y][x][Dpad]
+// volume: no LDS round trip stands between the chain and the store (an LDS
+// transpose into 16-byte stores took 71.7 us against 69.2 us at 848x480x128).
 
 constexpr int kRowVBlocks = 16;      // forward blocks in VGPRs (64 registers)
 constexpr int kRowABlocks = 60;      // forward blocks in AGPRs (240 registers)
@@ -165,32 +167,38 @@ __device__ __forceinline__ void rr_coef8(uint32_t (&w)[8], const uint4* lr, int 
   for (int j = 0; j < 8; ++j) w[j] = lr[x0 + j + 1].w;
 }
 
+// Final value of column xc (the high half of its t2) straight to the volume:
+// lane l writes disparity l of its group, so a wave's store covers 128
+// contiguous bytes of the column.
+template <uint32_t XB>
+__device__ __forceinline__ void rr_put(const RowOut& o, int xc, uint32_t t2) {
+  *reinterpret_cast<DE_GLOBAL uint16_t*>(o.vrow + size_t(xc) * XB + 2u * o.lane) =
+      uint16_t(t2 >> 16);
+}
+
 // H-bwd over P blocks b, b-1, ..., b-P+1 from their packed forward words. The
-// next block's coefficients are read before the current block's tile writes;
-// the tiles alternate, and a block's 16-byte stores leave after the next
-// block's chain.
+// next block's coefficients are read while the current block's chain runs,
+// and each column's final values leave as one 2-byte store a lane (no LDS
+// round trip between the chain and the volume).
 template <int P, uint32_t XB>
 __device__ __forceinline__ uint32_t rr_back_blocks(const uint4 (&v)[P], uint32_t t2,
                                                    const uint4* lr, int b, const RowOut& o,
                                                    uint32_t K) {
   uint32_t wc[8], wn[8];
-  uint4 pend = make_uint4(0u, 0u, 0u, 0u);
   rr_coef8(wc, lr, 8 * b);
 #pragma unroll
   for (int k = 0; k < P; ++k) {
     if (k + 1 < P) rr_coef8(wn, lr, 8 * (b - k - 1));
-    uint16_t* const tile = o.tile + (k & 1) * 8 * kWave;
     uint32_t t[8];
     rr_chain_back(wc, v[k], t2, K, t);
     t2 = t[0];
+    DE_GLOBAL uint8_t* const p = o.vrow + size_t(8 * (b - k)) * XB + 2u * o.lane;
 #pragma unroll
-    for (int j = 7; j >= 0; --j) tile[j * kWave + o.lane] = uint16_t(t[j] >> 16);
-    if (k > 0) row_store8<XB>(o, b - k + 1, pend);
-    pend = *reinterpret_cast<const uint4*>(tile + o.lane * 8);
+    for (int j = 7; j >= 0; --j)
+      *reinterpret_cast<DE_GLOBAL uint16_t*>(p + j * XB) = uint16_t(t[j] >> 16);
 #pragma unroll
     for (int j = 0; j < 8; ++j) wc[j] = wn[j];
   }
-  row_store8<XB>(o, b - P + 1, pend);
   return t2;
 }
 
@@ -231,10 +239,11 @@ __device__ __forceinline__ uint32_t rr_back_range(const LD& ld, uint32_t t2, con
 }
 
 // LDS bytes of a row-pass workgroup: table | left records | right records,
-// then per wave two transpose tiles and nl forward slots of 1 KB.
+// then per wave nl forward slots of 1 KB.
 __host__ __device__ inline size_t row_ws_fixed(int W, int T, int G) {
+  (void)G;
   const size_t tb = (size_t(T + 1) * kTblRow + 15) & ~size_t(15);
-  return tb + 2 * size_t(W + kRowPad) * 16 + size_t(G) * 2048;
+  return tb + 2 * size_t(W + kRowPad) * 16;
 }
 
 // G: disparity groups of every frame in the launch; P: H-bwd prefetch depth
@@ -278,9 +287,8 @@ void k_row(BatchView b, int base, int nl) {
   const int tb = (ntt * 2 + 15) & ~15;
   uint4* lr = reinterpret_cast<uint4*>(smem + tb);
   uint4* rrec = lr + (W + kRowPad);
-  uint8_t* const wbase = reinterpret_cast<uint8_t*>(rrec + (W + kRowPad)) + g * (2048 + nl * 1024);
-  o.tile = reinterpret_cast<uint16_t*>(wbase);
-  uint8_t* const lf = wbase + 2048;                      // this wave's forward slots
+  uint8_t* const lf = reinterpret_cast<uint8_t*>(rrec + (W + kRowPad)) + g * (nl * 1024);
+  o.tile = nullptr;                                      // stores go straight to the volume
   {
     // entry (a, h) = score of census distance h, intensity difference a <= T;
     // the frame's table holds it at (h << sc_shift) + a
@@ -454,11 +462,9 @@ void k_row(BatchView b, int base, int nl) {
     const uint32_t fv = xc >= xa ? load_col(xc) : uint32_t(kOff);
     const uint32_t w = lr[xc + 1].w;
     t2 = rf_chain_z(w, t2, mad_lo16(w, fv, K));
-    o.tile[(xc & 7) * kWave + lane] = uint16_t(t2 >> 16);
-    if ((xc & 7) == 0) row_flush<XB, true>(o, xc >> 3);
+    rr_put<XB>(o, xc, t2);
   };
-  o.tile[((W - 1) & 7) * kWave + lane] = uint16_t(t2 >> 16);
-  if (((W - 1) & 7) == 0) row_flush<XB, true>(o, (W - 1) >> 3);
+  rr_put<XB>(o, W - 1, t2);
   for (x = W - 2; x >= 8 * B1; --x) back1(x);            // columns from the aligned range's end
   if (nreg > 0) {
     uint4 vr[NV];                                        // block B1 - 1 - k first

@@ -160,6 +160,14 @@ class Plan:
     def launch(self, stream: int = 0):
         check(lib().de_plan_launch(self.h, C.c_void_p(stream) if stream else None))
 
+    @property
+    def parts(self) -> int:
+        """Sub-batches the plan pipelines (1: one graph of every stage)."""
+        n = lib().de_plan_parts(self.h)
+        if n < 0:
+            check(n)
+        return n
+
     def close(self):
         if self.h:
             lib().de_plan_destroy(self.h)

@@ -169,15 +169,26 @@ int de_dense_batch_dev(de_ctx* ctx, const de_pair* pairs, const de_cfg* cfgs, in
                        void* stream);
 
 /* A captured batch: its own resident scratch, descriptors uploaded once and
- * the stage launches recorded in a hipGraph. Launching replays the graph on
+ * the stage launches recorded in hipGraphs. Launching replays them on
  * `stream`; the caller refreshes the pairs' input buffers before each launch
  * (e.g. hipMemcpyAsync on the same stream). Plans of one context may be in
- * flight concurrently on different streams. */
+ * flight concurrently on different streams; one plan runs one launch at a
+ * time (launches queued on one stream follow each other).
+ * With DE_PLAN_PARTS = k (2..8) in the environment at creation, the batch is
+ * split into k parts of equal px*disp work: part i's prep and row pass run on
+ * one internal stream in part order, its column pass and solver on a second
+ * one after its own row pass, so one part's column pass overlaps the next
+ * part's row pass. The default is one graph of every stage. The results are
+ * the same bit for bit either way.
+ * (No reference counterpart: the reference tool runs one pair at a time,
+ * core/tools/de_dense_cuda.cu:670-770.) */
 typedef struct de_plan de_plan;
 int de_plan_create(de_ctx* ctx, const de_pair* pairs, const de_cfg* cfgs, int n,
                    de_plan** out);
 int de_plan_launch(de_plan* plan, void* stream);
 void de_plan_destroy(de_plan* plan);
+/* Number of parts of a plan (1: one graph of every stage). */
+int de_plan_parts(const de_plan* plan);
 
 /* Autotuner: times every prefetch-depth variant of the row and column passes
  * on a synthetic pair of this shape (reps launches each), remembers the

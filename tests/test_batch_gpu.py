@@ -112,3 +112,56 @@ def test_variants_do_not_change_results(ctx):
     rv, cv = ctx.autotune(424, 240, cfg, reps=3)
     assert 0 <= rv < 4 and 0 <= cv < 3
     assert all(same(a, b) for a, b in zip(ctx.run_dense(L, R, cfg, want_cand=True), ref))
+
+
+C4_SIZES = [(320, 240), (424, 240), (640, 360), (848, 480), (1280, 720)]
+C4_DISP = [64, 96, 128, 192]
+
+
+@pytest.mark.parametrize("parts", [1, 4])
+def test_plan_c4_mix_equals_per_pair(ctx, parts, monkeypatch):
+    """The 64-pair mix of uneven sizes and disparity ranges (census 9x7, 8
+    iterations) as one plan (one graph, or pipelined in 4 parts), launched
+    twice with new inputs, against per-pair plain launches."""
+    monkeypatch.setenv("DE_PLAN_PARTS", str(parts))
+    from doubleeye_amd import synth
+    from doubleeye_amd.dense import Cfg, Pair
+    rng = np.random.default_rng(4)
+    draws = [(C4_SIZES[int(rng.integers(len(C4_SIZES)))], C4_DISP[int(rng.integers(len(C4_DISP)))])
+             for _ in range(64)]
+    items = []
+    for i, ((W, H), D) in enumerate(draws):
+        t = dict(L=ctx.empty((H, W), np.uint8), R=ctx.empty((H, W), np.uint8),
+                 disp=ctx.empty((H, W), np.float32), margin=ctx.empty((H, W), np.float32),
+                 pd=ctx.empty((H, W), np.float32), pm=ctx.empty((H, W), np.float32),
+                 W=W, H=H, D=D, cfg=Cfg(dmax=D, iters=8, sigma_s=20.0, sigma_r=0.0392,
+                                        census_hw=4, census_hh=3))
+        t["pair"] = Pair(t["L"].ptr, t["R"].ptr, W, H, t["disp"].ptr, t["margin"].ptr)
+        items.append(t)
+    note("[c4] plan of 64 pairs")
+    plan = ctx.plan([t["pair"] for t in items], [t["cfg"] for t in items])
+    try:
+        note(f"[c4] parts {plan.parts}")
+        assert plan.parts == parts
+        for rep in range(2):
+            for i, t in enumerate(items):
+                L, R, _ = synth.pair_for(t["W"], t["H"], t["D"], 300 + 64 * rep + i,
+                                         kind="fronto" if (i + rep) % 2 else "slanted")
+                t["L"].upload(L)
+                t["R"].upload(R)
+            note(f"[c4] launch {rep}")
+            plan.launch(ctx.stream)
+            ctx.sync()
+            for i, t in enumerate(items):
+                ctx.run_dense_dev(t["L"].ptr, t["R"].ptr, t["W"], t["H"], t["cfg"], t["pd"].ptr,
+                                  t["pm"].ptr)
+            ctx.sync()
+            for i, t in enumerate(items):
+                assert same(t["disp"].numpy(), t["pd"].numpy()), \
+                    f"launch {rep} pair {i} {t['W']}x{t['H']}x{t['D']}: disparity differs"
+                assert same(t["margin"].numpy(), t["pm"].numpy()), f"launch {rep} pair {i}: margin"
+    finally:
+        plan.close()
+        for t in items:
+            for k in ("L", "R", "disp", "margin", "pd", "pm"):
+                t[k].free()
