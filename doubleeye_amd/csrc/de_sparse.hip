@@ -341,6 +341,138 @@ __global__ __launch_bounds__(kCandWaves * kWave) void k_sp_cand(SpBatch b, int t
 }
 
 // ---------------------------------------------------------------------------
+// The decision of match.cpp:409-476 over LDS-resident edges (the solver's
+// compacted row-major order c, which orders edges as their index e = i*K + k
+// does): beliefs bel[c], scores s[c], right index ej[c], column lists colq,
+// row starts rowst. An edge is dead once its row or its column is taken, so
+// one flag per edge serves both passes' "still free" tests. The rules and
+// orders are those of the global form below (pass 1: row and column agree,
+// output in left-keypoint order; pass 2: rounds of locally first edges in
+// (belief desc, score desc, edge asc) order, output in that order). Returns
+// the number of matches written.
+__device__ int decide_lds(const SpDesc& f, de_match_stats& st, const float* bel, const float* sc,
+                          const uint16_t* colq, const uint16_t* rowst, const uint16_t* ej,
+                          uint8_t* dead, uint16_t* rpick, uint16_t* cpick, uint16_t* acc,
+                          const int* col_end, int* shi, int* sh_any, int* sh_n2) {
+  constexpr uint16_t kNone = 0xFFFF;
+  const int tid = threadIdx.x, nl = f.nl, nr = f.nr, K = f.K;
+  auto col_begin = [&](int j) { return j > 0 ? col_end[j - 1] : 0; };
+  auto edge_of = [&](int i, int c) { return i * K + (c - int(rowst[i])); };
+  auto row_of = [&](int c) {   // the largest i with rowst[i] <= c: then c < rowst[i + 1]
+    int lo = 0, hi = nl - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (int(rowst[mid]) <= c) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  };
+  auto before = [&](int a, int b) {
+    if (bel[a] != bel[b]) return bel[a] > bel[b];
+    if (sc[a] != sc[b]) return sc[a] > sc[b];
+    return a < b;
+  };
+  auto kill_row = [&](int i) {
+    for (int c = rowst[i]; c < rowst[i + 1]; ++c) dead[c] = 1;
+  };
+  auto kill_col = [&](int j) {
+    for (int q = col_begin(j); q < col_end[j]; ++q) dead[colq[q]] = 1;
+  };
+  // best eligible edge per row (first among equals) and per column (smaller
+  // edge among equals)
+  for (int i = tid; i < nl; i += kSpThreads) {
+    int pick = -1;
+    for (int c = rowst[i]; c < rowst[i + 1]; ++c) {
+      if (sc[c] <= f.lambda) continue;
+      if (pick < 0 || bel[c] > bel[pick]) pick = c;
+    }
+    rpick[i] = pick < 0 ? kNone : uint16_t(pick);
+  }
+  for (int j = tid; j < nr; j += kSpThreads) {
+    int pick = -1;
+    for (int q = col_begin(j); q < col_end[j]; ++q) {
+      const int c = colq[q];
+      if (sc[c] <= f.lambda) continue;
+      if (pick < 0 || bel[c] > bel[pick] || (bel[c] == bel[pick] && c < pick)) pick = c;
+    }
+    cpick[j] = pick < 0 ? kNone : uint16_t(pick);
+  }
+  __syncthreads();
+  // pass 1
+  int n1 = 0;
+  {
+    const int ch = (nl + kSpThreads - 1) / kSpThreads;
+    const int r0 = min(nl, tid * ch), r1 = min(nl, r0 + ch);
+    auto agreed = [&](int i) { const int c = rpick[i]; return c != kNone && cpick[ej[c]] == c; };
+    int cnt = 0;
+    for (int i = r0; i < r1; ++i) cnt += agreed(i) ? 1 : 0;
+    int pos = block_excl_scan(cnt, shi, &n1);
+    for (int i = r0; i < r1; ++i)
+      if (agreed(i)) {
+        const int c = rpick[i];
+        put_match(f, pos++, edge_of(i, c), bel[c]);
+      }
+    __syncthreads();
+    for (int i = r0; i < r1; ++i)
+      if (agreed(i)) {
+        kill_row(i);
+        kill_col(ej[rpick[i]]);
+      }
+  }
+  if (tid == 0) *sh_n2 = 0;
+  __syncthreads();
+  // pass 2
+  for (;;) {
+    for (int i = tid; i < nl; i += kSpThreads) {
+      int pick = -1;
+      for (int c = rowst[i]; c < rowst[i + 1]; ++c) {
+        if (dead[c] || sc[c] <= f.lambda) continue;
+        if (pick < 0 || before(c, pick)) pick = c;
+      }
+      rpick[i] = pick < 0 ? kNone : uint16_t(pick);
+    }
+    for (int j = tid; j < nr; j += kSpThreads) {
+      int pick = -1;
+      for (int q = col_begin(j); q < col_end[j]; ++q) {
+        const int c = colq[q];
+        if (dead[c] || sc[c] <= f.lambda) continue;
+        if (pick < 0 || before(c, pick)) pick = c;
+      }
+      cpick[j] = pick < 0 ? kNone : uint16_t(pick);
+    }
+    if (tid == 0) *sh_any = 0;
+    __syncthreads();
+    for (int i = tid; i < nl; i += kSpThreads) {
+      const int c = rpick[i];
+      if (c != kNone && cpick[ej[c]] == c) {
+        acc[atomicAdd(sh_n2, 1)] = uint16_t(c);
+        *sh_any = 1;
+      }
+    }
+    __syncthreads();
+    // the accepted edges' rows and columns leave (after every test of the round)
+    for (int i = tid; i < nl; i += kSpThreads) {
+      const int c = rpick[i];
+      if (c != kNone && cpick[ej[c]] == c) {
+        kill_row(i);
+        kill_col(ej[c]);
+      }
+    }
+    const int any = *sh_any;
+    __syncthreads();
+    if (!any) break;
+  }
+  // pass 2's output order: rank among its acceptances
+  const int n2 = *sh_n2;
+  for (int a = tid; a < n2; a += kSpThreads) {
+    const int c = acc[a];
+    int rank = 0;
+    for (int b = 0; b < n2; ++b) rank += before(acc[b], c) ? 1 : 0;
+    put_match(f, n1 + rank, edge_of(row_of(c), c), bel[c]);
+  }
+  return n1 + n2;
+}
+
+// ---------------------------------------------------------------------------
 // k_sp_solve: one workgroup per frame.
 
 __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restrict__ frames) {
@@ -348,8 +480,6 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
   __shared__ float shf[kSpWaves];
   __shared__ int shi[kSpWaves];
   __shared__ int sh_any, sh_n2;
-  __shared__ int tile_e[kSpThreads];
-  __shared__ float tile_b[kSpThreads], tile_s[kSpThreads];
   const SpDesc f = frames[blockIdx.x];
   const int tid = threadIdx.x, nl = f.nl, nr = f.nr, K = f.K;
 
@@ -569,6 +699,33 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
       }
     }
     __syncthreads();
+    const size_t dec_need = need + size_t(nl) * 2 + size_t(nr) * 2 + size_t(min(nl, nr)) * 2;
+    if (dec_need <= size_t(f.lds_dyn)) {
+      // the decision in LDS too: beliefs over rho's slots, the edges' right
+      // index and taken flags over beta's, picks and acceptances after rowst
+      for (int c = tid; c < E; c += kSpThreads) rho_s[c] = beta_s[c] + rho_s[c] - es_s[c];
+      __syncthreads();
+      uint16_t* ej_s = reinterpret_cast<uint16_t*>(beta_s);
+      uint8_t* dead = reinterpret_cast<uint8_t*>(ej_s + E);
+      for (int i = tid; i < nl; i += kSpThreads) {
+        const int c0 = rowst[i], n = rowst[i + 1] - c0;
+        for (int k = 0; k < n; ++k) {
+          ej_s[c0 + k] = uint16_t(f.ej[i * K + k]);
+          dead[c0 + k] = 0;
+        }
+      }
+      uint16_t* rpick = rowst + nl + 1;
+      uint16_t* cpick = rpick + nl;
+      uint16_t* acc = cpick + nr;
+      __syncthreads();
+      const int n = decide_lds(f, st, rho_s, es_s, colq, rowst, ej_s, dead, rpick, cpick, acc,
+                               col_end, shi, &sh_any, &sh_n2);
+      if (tid == 0) {
+        *f.n_out = n;
+        *f.stats = st;
+      }
+      return;
+    }
     for (int i = tid; i < nl; i += kSpThreads) {
       const int c0 = rowst[i], n = rowst[i + 1] - c0;
       for (int k = 0; k < n; ++k) {
@@ -763,6 +920,11 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
   // Output order of the completion: rank among the accepted edges, counted
   // against tiles of their keys staged in LDS.
   const int n2 = sh_n2;
+  // ranking tiles after the column ends (the LDS form's arrays are dead here)
+  uint8_t* const tiles = reinterpret_cast<uint8_t*>(col_end) + ((size_t(nr) * 4 + 15) & ~size_t(15));
+  int* const tile_e = reinterpret_cast<int*>(tiles);
+  float* const tile_b = reinterpret_cast<float*>(tile_e + kSpThreads);
+  float* const tile_s = tile_b + kSpThreads;
   for (int a0 = 0; a0 < n2; a0 += kSpThreads) {
     const int a = a0 + tid;
     const int e = a < n2 ? f.acc2[a] : 0;
