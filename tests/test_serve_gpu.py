@@ -194,3 +194,60 @@ def test_budget_and_back_pressure(ctx):
             pass
     finally:
         srv.close()
+
+
+def test_sixteen_streams_under_load_batch_and_equal_plain_calls(ctx):
+    """The C5 shape under load: 16 streams (8 dense, 8 keypoint-graph, sizes
+    and disparity ranges from the batching sets) each submitting as fast as
+    its slots allow for 3 s, on one HIP lane so that the device is the
+    bottleneck. Requests arriving while others run must leave in batches, and
+    every result must equal the plain call on its inputs."""
+    from doubleeye_amd.serve import Server
+    sizes = [(320, 240), (424, 240), (640, 360), (848, 480), (1280, 720)]
+    disp = [64, 96, 128, 192]
+    rng = np.random.default_rng(16)
+    spec = []
+    for k in range(16):
+        W, H = sizes[int(rng.integers(len(sizes)))]
+        D = disp[int(rng.integers(len(disp)))]
+        spec.append(("dense" if k % 2 == 0 else "knn", W, H, D, 2))
+    srv = Server(ctx, mem_budget=16 << 30, hip_streams=1, slots=4)
+    try:
+        streams = make_streams(ctx, srv, spec)
+        note(f"[serve16] {spec}")
+        by_sid = {st["sid"]: st for st in streams}
+
+        def take(timeout_ms):
+            out = []
+            for r in srv.poll(timeout_ms=timeout_ms):
+                assert r.status == 0, r.status
+                st = by_sid[r.stream_id]
+                res, want = r.fetch(), st["want"][r.tag % 2]
+                if st["kind"] == "dense":
+                    assert same_dense(res, want), (r.stream_id, r.tag)
+                else:
+                    assert np.array_equal(res.view(np.uint8), want.view(np.uint8)), \
+                        (r.stream_id, r.tag)
+                srv.release(r)
+                out.append(r)
+            return out
+        count = [0] * 16
+        sent, got = 0, []
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < 3.0:
+            for k, st in enumerate(streams):
+                L, R, kp = st["frames"][count[k] % 2]
+                if srv.submit(st["sid"], L, R, kp, tag=count[k]) is not None:
+                    count[k] += 1
+                    sent += 1
+            got += take(0)
+        while len(got) < sent:
+            assert time.perf_counter() - t0 < 60.0, f"only {len(got)} of {sent} results"
+            got += take(50)
+        sizes_seen = [r.batch_size for r in got]
+        note(f"[serve16] {sent} frames, mean batch {np.mean(sizes_seen):.2f}, "
+             f"max {max(sizes_seen)}")
+        assert len(got) == sent and sent > 200
+        assert np.mean(sizes_seen) > 1.5 and max(sizes_seen) >= 4
+    finally:
+        srv.close()
