@@ -327,15 +327,21 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
   const int W = f.W, D = f.D, dmin = f.dmin;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const size_t rb = size_t(y) * W * D;
-  const float* vol = f.vol + rb;
-  float* beta = f.beta + rb;
-  float* rho = f.rho + rb;
+  const float* __restrict__ vol = f.vol + rb;
+  float* __restrict__ beta = f.beta + rb;
+  float* __restrict__ rho = f.rho + rb;
   constexpr int nchunk = NC;
   const float keep = 1.f - f.damping;
-  for (int t = threadIdx.x; t < W * D; t += kSolveT) {
-    beta[t] = 0.f;
-    rho[t] = 0.f;
-  }
+  // Messages start at zero. With iterations, nothing is filled: iteration 0
+  // takes its old beta and rho as the literal 0.0f a fill would have stored
+  // (the same operands, so the same bits), and every element of a pixel's
+  // window is written before it is read again (the window keeps x - dmin - k
+  // >= 3, so each has a diagonal).
+  if (f.iters == 0)
+    for (int t = threadIdx.x; t < W * D; t += kSolveT) {
+      beta[t] = 0.f;
+      rho[t] = 0.f;
+    }
   auto dk = [&](int set, int which) { return claim + size_t(set * 2 + which) * W; };
   for (int t = threadIdx.x; t < 2 * W; t += kSolveT) claim[t] = 0ull;   // set 0
   // margins (scores above -1e29 only)
@@ -358,13 +364,13 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
     }
   }
   __syncthreads();
-  // beta(x, k) from its diagonal's top-2 in key set `set` (the beta half-step)
-  auto beta_update = [&](int set, int x, int k, size_t i) {
+  // beta(x, k) from its diagonal's top-2 in key set `set` (the beta half-step),
+  // given the score s and the old beta b0; returns the new value
+  auto beta_new = [&](int set, int x, int k, float s, float b0) {
     const int xr = x - dmin - k;
-    if (xr < 0) return;
     const Diag t = diag_top2(dk(set, 0)[xr], dk(set, 1)[xr]);
     const float ex = (k == t.i1) ? t.b2 : t.b1;
-    beta[i] = keep * (vol[i] - fmax_ref(f.gamma, ex)) + f.damping * beta[i];
+    return keep * (s - fmax_ref(f.gamma, ex)) + f.damping * b0;
   };
   for (int it = 0; it < f.iters; ++it) {
     const int cs = it & 1, ps = cs ^ 1;
@@ -374,15 +380,27 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
       int lo, hi;
       window(x, W, D, dmin, &lo, &hi);
       if (hi <= lo) continue;
-      float v[kMaxChunks];
+      float v[kMaxChunks], sv[kMaxChunks], r0[kMaxChunks];
       bool ok[kMaxChunks];
+      // every operand of the pixel's elements is loaded once
       #pragma unroll
       for (int c = 0; c < nchunk; ++c) {
         const int k = c * kWave + lane;
         ok[c] = k >= lo && k < hi;
         const size_t i = size_t(x) * D + k;
-        if (ok[c] && it > 0) beta_update(ps, x, k, i);
-        v[c] = ok[c] ? beta[i] : 0.f;
+        sv[c] = ok[c] ? vol[i] : 0.f;
+        // beta is first written at iteration 1, rho at iteration 0
+        v[c] = (ok[c] && it > 1) ? beta[i] : 0.f;
+        r0[c] = (ok[c] && it > 0) ? rho[i] : 0.f;
+      }
+      if (it > 0) {
+        #pragma unroll
+        for (int c = 0; c < nchunk; ++c) {
+          if (!ok[c]) continue;
+          const int k = c * kWave + lane;
+          v[c] = beta_new(ps, x, k, sv[c], v[c]);
+          beta[size_t(x) * D + k] = v[c];
+        }
       }
       const WTop2 t = wave_top2(v, ok, nchunk, lane);
       #pragma unroll
@@ -391,8 +409,8 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
         const int k = c * kWave + lane;
         const size_t i = size_t(x) * D + k;
         const float ex = (k == t.i1) ? t.b2 : t.b1;
-        const float tgt = vol[i] - fmax_ref(f.lambda, ex);
-        const float r = keep * tgt + f.damping * rho[i];
+        const float tgt = sv[c] - fmax_ref(f.lambda, ex);
+        const float r = keep * tgt + f.damping * r0[c];
         rho[i] = r;
         const int xr = x - dmin - k;
         if (xr >= 0) {
@@ -410,7 +428,7 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
     for (int t = threadIdx.x; t < 2 * W; t += kSolveT) dk(ps, 0)[t] = 0ull;
     __syncthreads();
   }
-  // the last iteration's beta half-step
+  // the last iteration's beta half-step (old beta: zero after one iteration)
   if (f.iters > 0) {
     const int ls = (f.iters - 1) & 1;
     for (int x = wv; x < W; x += kSolveWaves) {
@@ -419,7 +437,9 @@ __global__ __launch_bounds__(kSolveT) void k_f32_solve(F32Desc f) {
       #pragma unroll
       for (int c = 0; c < nchunk; ++c) {
         const int k = c * kWave + lane;
-        if (k >= lo && k < hi) beta_update(ls, x, k, size_t(x) * D + k);
+        if (k < lo || k >= hi) continue;
+        const size_t i = size_t(x) * D + k;
+        beta[i] = beta_new(ls, x, k, vol[i], f.iters > 1 ? beta[i] : 0.f);
       }
     }
     __syncthreads();
