@@ -10,7 +10,9 @@ over the timed region). Frames run as a video stream would, several in flight
 captured plan (its own scratch and hipGraph) on its own HIP stream, launched
 in turn, so one frame's row pass overlaps another's column pass (the
 reference GPU tool pipelines two frames, core/tools/de_dense_cuda.cu:788-841);
-every frame runs all four stages and yields the same output.
+every frame runs all four stages and yields the same output. Frames in flight
+use the streaming row pass (two waves a SIMD), a lone frame the
+register-resident one (see measure_c2).
 `single_stream_ms_per_step` is the same run with one frame at a time. `roofline` is for the dominant kernel, timed live with
 hipEvents recorded between the stage launches on the launching stream (one
 frame at a time, so the kernel runs alone);
@@ -37,6 +39,7 @@ CFG7 = dict(dmax=128, iters=8, sigma_s=20.0, sigma_r=0.0392)
 CFG = dict(CFG7, census_hw=4, census_hh=3)                 # C2 as stated: census 9x7
 REF_FLAGS = ["--dmax", "128", "--iters", "8", "--sigma-s", "20", "--sigma-r", "0.0392"]
 HBM_PEAK_GBS = 8000.0
+ROW_STREAMING = 2          # row-pass launch variant: streaming, backward prefetch 8 blocks
 # HBM bytes per launch of each stage kernel at C2 (census 9x7), from rocprofv3
 # FETCH_SIZE (x2) and WRITE_SIZE (x1) runs on this workload, the factors read off
 # a 256 MiB device copy in the same runs: a quoted profile figure, not measured
@@ -1085,7 +1088,7 @@ def main():
         "config": {"workload": "dense MASDA 848x480, D=128 (d=1..128), census 9x7, "
                                "sigma_s 20, sigma_r 10/255, 8 BP iterations, top-2",
                    "W": W, "H": H, "D": D, "iters": 8, "census": "9x7",
-                   "frames_in_flight": nfl},
+                   "frames_in_flight": nfl, "in_flight_row_pass": m9["inflight_row_pass"]},
         "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -1157,9 +1160,18 @@ def measure_c2(ctx, extra, dL, dR, cfg_kw: dict, a) -> dict:
     ctx.sync()
     first = disp.numpy()
     slots = []
+    # Frames in flight take the streaming row pass (about 210 registers a
+    # lane, so the row passes of two frames share a SIMD); one frame alone
+    # takes the library's choice, the register-resident pass (512 registers a
+    # lane, one wave a SIMD), which is the faster one for a lone frame.
+    # Measured at 848x480x128: resident 0.133 / 0.152 / 0.143 ms a frame with
+    # 2 / 3 / 4 in flight, streaming 0.149 / 0.132 / 0.129.
+    inflight_rv = a.row_variant if a.row_variant >= 0 else (ROW_STREAMING if nfl > 2 else -1)
+    ctx.set_variants(inflight_rv, -1)
     for _ in range(nfl):
         o = (ctx.empty((H, W), np.float32), ctx.empty((H, W), np.float32))
         slots.append((ctx.plan([Pair(dL.ptr, dR.ptr, W, H, o[0].ptr, o[1].ptr)], [cfg]), o))
+    ctx.set_variants(a.row_variant, -1)
     streams = [ctx.stream] + [c.stream for c in extra]
 
     def run_inflight(k_steps):
@@ -1203,6 +1215,8 @@ def measure_c2(ctx, extra, dL, dR, cfg_kw: dict, a) -> dict:
         o[0].free(); o[1].free()
     disp.free(); margin.free()
     return {"ms": ms, "ms_single": ms_single, "stage_ms": per, "deterministic": deterministic,
+            "inflight_row_pass": {-1: "library's choice", 0: "resident", 1: "resident",
+                                  2: "streaming", 3: "streaming"}[inflight_rv],
             "coverage": round(float(np.isfinite(last).mean()), 4)}
 
 
