@@ -998,6 +998,9 @@ def main():
     ap.add_argument("--row-variant", type=int, default=-1,
                     help="c2, c4: force the row pass's launch variant (0-3; -1: the "
                          "library's choice; c4: the batch's plan only)")
+    ap.add_argument("--col-variant", type=int, default=-1,
+                    help="c2: force the column pass's launch variant (0-2: 8, 16, 32 rows in "
+                         "flight; -1: the library's choice)")
     ap.add_argument("--c5-seconds", type=float, default=20.0,
                     help="c2: seconds of the 16-stream serving leg reported beside the "
                          "headline (p50/p99 latency); 0 skips it")
@@ -1062,8 +1065,8 @@ def main():
         ctx.sync()
         src.free(); dst.free()
     dL, dR = ctx.to_device(L), ctx.to_device(R)
-    if a.row_variant >= 0:
-        ctx.set_variants(a.row_variant, -1)
+    if a.row_variant >= 0 or a.col_variant >= 0:
+        ctx.set_variants(a.row_variant, a.col_variant)
 
     m9 = measure_c2(ctx, extra, dL, dR, CFG, a)
     m7 = measure_c2(ctx, extra, dL, dR, CFG7, a) if a.census == "both" else None
@@ -1167,11 +1170,11 @@ def measure_c2(ctx, extra, dL, dR, cfg_kw: dict, a) -> dict:
     # Measured at 848x480x128: resident 0.133 / 0.152 / 0.143 ms a frame with
     # 2 / 3 / 4 in flight, streaming 0.149 / 0.132 / 0.129.
     inflight_rv = a.row_variant if a.row_variant >= 0 else (ROW_STREAMING if nfl > 2 else -1)
-    ctx.set_variants(inflight_rv, -1)
+    ctx.set_variants(inflight_rv, a.col_variant)
     for _ in range(nfl):
         o = (ctx.empty((H, W), np.float32), ctx.empty((H, W), np.float32))
         slots.append((ctx.plan([Pair(dL.ptr, dR.ptr, W, H, o[0].ptr, o[1].ptr)], [cfg]), o))
-    ctx.set_variants(a.row_variant, -1)
+    ctx.set_variants(a.row_variant, a.col_variant)
     streams = [ctx.stream] + [c.stream for c in extra]
 
     def run_inflight(k_steps):

@@ -97,11 +97,13 @@ void k_col(BatchView b, int base) {
 #pragma unroll
     for (int u = 0; u < U; ++u) v[u] = __builtin_amdgcn_raw_buffer_load_b16(rr0, vo, u * ysb, 0);
   };
+  // coefficients of rows y0 .. y0+U-1: one wide scalar load, issued a chunk
+  // ahead of its use like the rows themselves (a scalar load waited on at
+  // once costs the whole memory latency every U rows)
+  auto ldc = [&](uint32_t (&c)[U], int y0) { __builtin_memcpy(c, ay + y0, sizeof(c)); };
   {
-    auto fwd = [&](const uint32_t (&v)[U], int y0, int n) {
+    auto fwd = [&](const uint32_t (&v)[U], const uint32_t (&ab)[U], int y0, int n) {
       const __amdgpu_buffer_rsrc_t rr0 = rs(y0);
-      uint32_t ab[U];                                    // coefficients of the chunk, one
-      __builtin_memcpy(ab, ay + y0, sizeof(ab));         // wide scalar load
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (u < n) {
@@ -112,38 +114,42 @@ void k_col(BatchView b, int base) {
       }
     };
     const int yend = NRB > 0 ? yreg : H;
-    uint32_t A[U], B[U];
+    uint32_t A[U], B[U], cA[U], cB[U];
     int y = 1;
     ldf(A, y);
+    ldc(cA, y);
     for (; y + 2 * U <= yend; y += 2 * U) {
       ldf(B, y + U);
-      fwd(A, y, U);
+      ldc(cB, y + U);
+      fwd(A, cA, y, U);
       ldf(A, y + 2 * U);
-      fwd(B, y + U, U);
+      ldc(cA, y + 2 * U);
+      fwd(B, cB, y + U, U);
     }
     if (y + U <= yend) {
       ldf(B, y + U);
-      fwd(A, y, U);
-      fwd(B, y + U, yend - y - U);
+      ldc(cB, y + U);
+      fwd(A, cA, y, U);
+      fwd(B, cB, y + U, yend - y - U);
     } else {
-      fwd(A, y, yend - y);
+      fwd(A, cA, y, yend - y);
     }
   }
   if constexpr (NRB > 0) {
     // ---- V-fwd over the register rows yreg .. H-2, then row H-1 ----
-    uint32_t V[2][U];
+    uint32_t V[2][U], AB[2][U];
     ldf(V[0], yreg);
+    ldc(AB[0], yreg);
     uint32_t hold = 0u;                                  // odd row's half, waiting for its pair
     static_for<NRR / U>([&](auto CC) __attribute__((always_inline)) {
       constexpr int c = decltype(CC)::value;
       const int y0 = yreg + c * U;
       ldf(V[(c + 1) & 1], c + 1 < NRR / U ? y0 + U : H - 1);   // the last brings row H-1
-      uint32_t ab[U];
-#pragma unroll
-      for (int i = 0; i < U; ++i) ab[i] = ay[y0 + i];     // wide scalar loads
+      if constexpr (c + 1 < NRR / U) ldc(AB[(c + 1) & 1], y0 + U);
+      __builtin_amdgcn_sched_barrier(0);                 // the loads stay ahead of this chunk
       static_for<U>([&](auto UU) __attribute__((always_inline)) {
         constexpr int u = decltype(UU)::value;
-        const uint32_t a2 = ab[u];
+        const uint32_t a2 = AB[c & 1][u];
         t = rf_chain(a2, t, rf_acc(a2, V[c & 1][u]));
         constexpr int j = NRR - 1 - (c * U + u);         // row y0 + u = H-2-j; odd j first
         if constexpr (j & 1) {
@@ -179,11 +185,16 @@ void k_col(BatchView b, int base) {
     if (STORE) __builtin_amdgcn_raw_buffer_store_b16(uint16_t(vv), rr0, vo, (U - 1 - u) * ysb, 0);
     rb16[ru * (2 * RS) + slot] = uint16_t(vv);
   };
-  auto bwd = [&](const uint32_t (&v)[U], int ytop, int u0, int n) {
+  // coefficients of the chunk of rows ytop .. ytop-U+1: row ytop-u uses the
+  // coefficient of row ytop-u+1 = c[U-1-u] (one wide scalar load, issued a
+  // chunk ahead; chunks run down by U rows, through the register blocks and on
+  // into the volume's, so a chunk's successor is ytop - U; reads below row 0
+  // stay in the plane's padding)
+  auto ldcb = [&](uint32_t (&c)[U], int ytop) {
+    __builtin_memcpy(c, ay + (ytop - U + 2), sizeof(c));
+  };
+  auto bwd = [&](const uint32_t (&v)[U], const uint32_t (&ab)[U], int ytop, int u0, int n) {
     const __amdgpu_buffer_rsrc_t rr0 = rs(ytop - U + 1);
-    // coefficient of row ytop-u+1 = ab[U-1-u]: one wide scalar load per chunk
-    uint32_t ab[U];
-    __builtin_memcpy(ab, ay + (ytop - U + 2), sizeof(ab));
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (u < n) bwd1(v[u], ab[U - 1 - u], rr0, u, u0 + u);
@@ -258,18 +269,23 @@ void k_col(BatchView b, int base) {
     }
     __syncthreads();
   };
+  uint32_t cA[U], cB[U];                                // coefficient chunks in flight
   if constexpr (NRB > 0) {
     // the register blocks: block k holds rows H-2-64k .. H-65-64k, slots 64k ..
+    uint32_t CR[2][U];
+    ldcb(CR[0], H - 2);
     static_for<NRB>([&](auto KK) __attribute__((always_inline)) {
       constexpr int k = decltype(KK)::value;
       const int yb = H - 2 - kWave * k;
       static_for<kWave / U>([&](auto QQ) __attribute__((always_inline)) {
         constexpr int q = decltype(QQ)::value;
+        constexpr int n = k * (kWave / U) + q;           // chunk number from the top
         const int ytop = yb - q * U;
         const __amdgpu_buffer_rsrc_t rr0 = rs(ytop - U + 1);
-        uint32_t ab[U];
-#pragma unroll
-        for (int i = 0; i < U; ++i) ab[i] = ay[ytop - U + 2 + i];
+        ldcb(CR[(n + 1) & 1], ytop - U);                 // the next chunk's (the last
+                                                         // one's: the volume's first)
+        __builtin_amdgcn_sched_barrier(0);               // issued before this chunk's work
+        const uint32_t (&ab)[U] = CR[n & 1];
         static_for<U / 2>([&](auto PP) __attribute__((always_inline)) {
           constexpr int u = 2 * decltype(PP)::value;
           constexpr int j = kWave * k + q * U + u;       // even: rows j, j + 1 share a word
@@ -282,6 +298,11 @@ void k_col(BatchView b, int base) {
       });
       top2_block(yb, kWave);
     });
+    constexpr int nlast = NRB * (kWave / U);             // chunks done: the next is in
+#pragma unroll                                           // CR[nlast & 1]
+    for (int i = 0; i < U; ++i) cA[i] = CR[nlast & 1][i];
+  } else {
+    ldcb(cA, H - 2);
   }
   const int yb0 = H - 2 - NRR;                           // first block from the volume
   uint32_t A[U], B[U];
@@ -291,17 +312,20 @@ void k_col(BatchView b, int base) {
     int u0 = 0;
     for (; u0 + 2 * U <= nrow; u0 += 2 * U) {
       ldb(B, yb - u0 - U);
-      bwd(A, yb - u0, u0, U);
+      ldcb(cB, yb - u0 - U);
+      bwd(A, cA, yb - u0, u0, U);
       ldb(A, yb - u0 - 2 * U);
-      bwd(B, yb - u0 - U, u0 + U, U);
+      ldcb(cA, yb - u0 - 2 * U);
+      bwd(B, cB, yb - u0 - U, u0 + U, U);
     }
     if (u0 < nrow) {                                     // last block: fewer than 2U rows left
       if (u0 + U <= nrow) {
         ldb(B, yb - u0 - U);
-        bwd(A, yb - u0, u0, U);
-        bwd(B, yb - u0 - U, u0 + U, nrow - u0 - U);
+        ldcb(cB, yb - u0 - U);
+        bwd(A, cA, yb - u0, u0, U);
+        bwd(B, cB, yb - u0 - U, u0 + U, nrow - u0 - U);
       } else {
-        bwd(A, yb - u0, u0, nrow - u0);
+        bwd(A, cA, yb - u0, u0, nrow - u0);
       }
     }
     top2_block(yb, nrow);

@@ -100,6 +100,31 @@ __device__ __forceinline__ uint32_t row_tbl(uint32_t hh, uint32_t adb) {
       uintptr_t((hh << 1) + adb));
 }
 
+// Stage the score table for the row passes: entry (a, h) = score of census
+// distance h, intensity difference a <= T, at i = a * (kTblRow / 2) + h; the
+// frame's table holds it at (h << sc_shift) + a. Threads tid, tid + nt, ...
+// issue their loads kStage at a time before storing any (a load's latency is
+// paid once per group, not once per entry).
+constexpr int kStage = 8;
+__device__ __forceinline__ void stage_score_table(uint16_t* tbl, const DE_CONST uint16_t* sc,
+                                                  int ntt, int nb, int sc_shift, int tid,
+                                                  int nt) {
+  for (int i0 = tid; i0 < ntt; i0 += kStage * nt) {
+    uint16_t v[kStage];
+    bool ok[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int i = i0 + u * nt;
+      const int a = i / int(kTblRow / 2), h = i - a * int(kTblRow / 2);
+      ok[u] = i < ntt && h <= nb;
+      v[u] = sc[ok[u] ? (h << sc_shift) + a : 0];   // entry 0 stands in: loads unconditional
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; ++u)
+      if (i0 + u * nt < ntt) tbl[i0 + u * nt] = ok[u] ? v[u] : uint16_t(kOff);
+  }
+}
+
 __device__ __forceinline__ uint32_t row_score(const RowLds& q, const uint4& rec, uint64_t cr,
                                               uint32_t rb) {
   const uint32_t h = bcnt_acc(rec.y ^ uint32_t(cr >> 32), __popc(rec.x ^ uint32_t(cr)) + q.hb);

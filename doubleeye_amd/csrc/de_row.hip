@@ -293,23 +293,38 @@ void k_row(BatchView b, int base, int nl) {
     // entry (a, h) = score of census distance h, intensity difference a <= T;
     // the frame's table holds it at (h << sc_shift) + a
     uint16_t* tbl = reinterpret_cast<uint16_t*>(smem);
-    const DE_CONST uint16_t* sc = cptr(F.sc);
-    for (int i = threadIdx.x; i < ntt; i += 64 * G) {
-      const int a = i / int(kTblRow / 2), h = i - a * int(kTblRow / 2);
-      tbl[i] = h <= nb ? sc[(h << F.sc_shift) + a] : uint16_t(kOff);
-    }
+    stage_score_table(tbl, cptr(F.sc), ntt, nb, F.sc_shift, threadIdx.x, 64 * G);
     const DE_CONST uint4* lrg = cptr(reinterpret_cast<const uint4*>(F.lrec)) + size_t(y) * W;
     const DE_CONST uint64_t* crg = cptr(F.cr) + size_t(y) * W;
     const DE_CONST uint8_t* Rg = cptr(F.R) + size_t(y) * W;
-    for (int x = threadIdx.x; x < W + kRowPad; x += 64 * G) {
-      uint4 rec = make_uint4(0u, 0u, 0u, 0u), rv = make_uint4(0u, 0u, 0u, 0u);
-      if (x < W) {
-        rec = lrg[x];
-        const uint64_t c = crg[x];
-        rv = make_uint4(uint32_t(c), uint32_t(c >> 32), kTblRow * Rg[x], 0u);
+    // kStage columns a thread in flight: every load of the group is issued
+    // before the first LDS store (one memory latency a group, not a column)
+    constexpr int nt = 64 * G;
+    for (int x0 = threadIdx.x; x0 < W + kRowPad; x0 += kStage * nt) {
+      uint4 rec[kStage];
+      uint64_t c[kStage];
+      uint32_t r[kStage];
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int x = x0 + u * nt;
+        const bool in = x < W;
+        rec[u] = make_uint4(0u, 0u, 0u, 0u);
+        c[u] = 0ull;
+        r[u] = 0u;
+        if (in) {
+          rec[u] = lrg[x];
+          c[u] = crg[x];
+          r[u] = Rg[x];
+        }
       }
-      lr[x] = rec;
-      rrec[x] = rv;
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int x = x0 + u * nt;
+        if (x < W + kRowPad) {
+          lr[x] = rec[u];
+          rrec[x] = make_uint4(uint32_t(c[u]), uint32_t(c[u] >> 32), kTblRow * r[u], 0u);
+        }
+      }
     }
   }
   __syncthreads();
@@ -643,11 +658,7 @@ __global__ __launch_bounds__(256) void k_rows(BatchView b, int base, int items, 
     // entry (a, h) = score of census distance h, intensity difference a <= T;
     // the frame's table holds it at (h << sc_shift) + a
     uint16_t* tbl = reinterpret_cast<uint16_t*>(smem);
-    const DE_CONST uint16_t* sc = cptr(F.sc);
-    for (int i = lane; i < ntt; i += kWave) {
-      const int a = i / int(kTblRow / 2), h = i - a * int(kTblRow / 2);
-      tbl[i] = h <= nb ? sc[(h << F.sc_shift) + a] : uint16_t(kOff);
-    }
+    stage_score_table(tbl, cptr(F.sc), ntt, nb, F.sc_shift, lane, kWave);
     const DE_CONST uint4* lrg = cptr(reinterpret_cast<const uint4*>(F.lrec)) + size_t(y) * W;
     const DE_CONST uint64_t* crg = cptr(F.cr) + size_t(y) * W;
     const DE_CONST uint8_t* Rg = cptr(F.R) + size_t(y) * W;
