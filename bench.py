@@ -44,7 +44,7 @@ ROW_STREAMING = 2          # row-pass launch variant: streaming, backward prefet
 # FETCH_SIZE (x2) and WRITE_SIZE (x1) runs on this workload, the factors read off
 # a 256 MiB device copy in the same runs: a quoted profile figure, not measured
 # in this run; the source file is named in the output line.
-TRAFFIC = {"prep": 15_535_071, "row": 125_931_507, "col": 190_547_754, "solve": 9_888_890}
+TRAFFIC = {"prep": 15_534_080, "row": 122_968_064, "col": 190_605_312, "solve": 9_889_792}
 TRAFFIC_SOURCE = "quoted: profiles/dense_c2_census9x7_848x480x128_kernels.md"
 
 
