@@ -37,6 +37,7 @@ sys.path.insert(0, ROOT)
 W, H, D = 848, 480, 128
 CFG7 = dict(dmax=128, iters=8, sigma_s=20.0, sigma_r=0.0392)
 CFG = dict(CFG7, census_hw=4, census_hh=3)                 # C2 as stated: census 9x7
+CFG_C1 = dict(dmax=48, iters=5, sigma_s=20.0, sigma_r=0.0392, census_hw=4, census_hh=3)
 REF_FLAGS = ["--dmax", "128", "--iters", "8", "--sigma-s", "20", "--sigma-r", "0.0392"]
 HBM_PEAK_GBS = 8000.0
 ROW_STREAMING = 2          # row-pass launch variant: streaming, backward prefetch 8 blocks
@@ -196,13 +197,15 @@ def cpu_baseline(L, R, budget_s: float = 12.0):
                             "frames": len(mt)}}
 
 
-def cpu_baseline_port(L, R, budget_s: float = 12.0, cfg: dict | None = None):
+def cpu_baseline_port(L, R, budget_s: float = 12.0, cfg: dict | None = None, w: int = W,
+                      h: int = H):
     """The oracle's restatement of the reference pipeline (single thread, the
     reference's arithmetic) on the same pair, repeated to ~budget_s. The
     reference's own tool has no 9x7 census, so the stated C2 workload is timed
     through the port."""
     from oracle import binding as orc
     cfg = CFG if cfg is None else cfg
+    d = cfg["dmax"] - cfg.get("dmin", 1) + 1
     c = orc.cfg(**cfg)
     times = []
     t_end = time.time() + budget_s
@@ -212,10 +215,11 @@ def cpu_baseline_port(L, R, budget_s: float = 12.0, cfg: dict | None = None):
         times.append((time.perf_counter() - t0) * 1e3)
     ms = sum(times) / len(times)
     win = f"{2 * cfg.get('census_hw', 3) + 1}x{2 * cfg.get('census_hh', 3) + 1}"
-    return {"value": round(W * H * D / (ms * 1e-3) / 1e6, 2), "unit": "Mpx*disp/s",
+    return {"value": round(w * h * d / (ms * 1e-3) / 1e6, 2), "unit": "Mpx*disp/s",
             "cores": 1, "kind": "port",
-            "sample": f"{len(times)} frames of 848x480x128 iters 8 census {win} through the "
-                      f"oracle (C++ restatement of de_dense --threads 1), mean {ms:.1f} ms/frame"}
+            "sample": f"{len(times)} frames of {w}x{h}x{d} iters {cfg['iters']} census {win} "
+                      f"through the oracle (C++ restatement of de_dense --threads 1), "
+                      f"mean {ms:.1f} ms/frame"}
 
 
 def bench_sparse(a):
@@ -1113,10 +1117,26 @@ def main():
                             "single_stream_ms_per_step": round(m7["ms_single"], 4),
                             "stage_ms": {n: round(v, 4) for n, v in m7["stage_ms"].items()},
                             "coverage": m7["coverage"]}
+    # C1 (the reference's CPU-runnable case): 424x240, d 1..48, 5 iterations,
+    # census 9x7, fronto planes at 8/20/35
+    L1, R1, _ = synth.fronto_pair(424, 240, seed=1, disparities=(8, 20, 35))
+    d1L, d1R = ctx.to_device(L1), ctx.to_device(R1)
+    m1 = measure_c2(ctx, extra, d1L, d1R, CFG_C1, a, W=424, H=240)
+    out["c1"] = {"workload": "dense MASDA 424x240, D=48 (d=1..48), census 9x7, 5 BP iterations",
+                 "value": round(424 * 240 * 48 / (m1["ms"] * 1e-3) / 1e6, 1),
+                 "unit": "Mpx*disp/s", "ms_per_step": round(m1["ms"], 4),
+                 "single_stream_ms_per_step": round(m1["ms_single"], 4),
+                 "stage_ms": {n: round(v, 4) for n, v in m1["stage_ms"].items()},
+                 "coverage": m1["coverage"], "deterministic": m1["deterministic"]}
+    d1L.free(); d1R.free()
     for c in extra:
         c.close()
     ctx.close()
     if not a.no_cpu_baseline:
+        out["c1"]["cpu_baseline"] = cpu_baseline_port(np.ascontiguousarray(L1),
+                                                       np.ascontiguousarray(R1),
+                                                       min(3.0, a.cpu_budget), CFG_C1,
+                                                       w=424, h=240)
         Lc, Rc = np.ascontiguousarray(L), np.ascontiguousarray(R)
         out["cpu_baseline"] = cpu_baseline_port(Lc, Rc, a.cpu_budget)
         if m7 is not None:
@@ -1144,11 +1164,11 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def measure_c2(ctx, extra, dL, dR, cfg_kw: dict, a) -> dict:
-    """One configuration of the C2 pair: frames in flight (a captured plan per
-    frame slot, slot k on HIP stream k), then one frame at a time, then the
-    per-stage kernel times from hipEvents between the launches on the
-    launching stream (one frame at a time, so each kernel runs alone)."""
+def measure_c2(ctx, extra, dL, dR, cfg_kw: dict, a, W: int = W, H: int = H) -> dict:
+    """One configuration of a W x H pair (C2 by default): frames in flight (a
+    captured plan per frame slot, slot k on HIP stream k), then one frame at a
+    time, then the per-stage kernel times from hipEvents between the launches
+    on the launching stream (one frame at a time, so each kernel runs alone)."""
     import numpy as np
 
     from doubleeye_amd.dense import Cfg, Pair
