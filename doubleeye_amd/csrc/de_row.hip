@@ -440,24 +440,31 @@ void k_row(BatchView b, int base, int nl) {
         rr_score_raw(R[0], S[0], Tb);
       }
     }
-    static_for<NT>([&](auto J) __attribute__((always_inline)) {
+    auto slot_fwd = [&](auto J) __attribute__((always_inline)) {
       constexpr int j = decltype(J)::value;
-      if (j >= j0) {
-        const int xj = 8 * (Bb + j);
-        if constexpr (j + 2 < NT) rr_load_raw(lr, rrec, R[j & 1], xj + 16, d);
-        if constexpr (j + 1 < NT) rr_score_raw(R[(j + 1) & 1], S[(j + 1) & 1], Tb);
-        uint4 v;
-        t2 = rr_chain(S[j & 1], t2, K, v);
-        if constexpr (j < NA) {
-          ag[4 * j] = to_acc(v.x);
-          ag[4 * j + 1] = to_acc(v.y);
-          ag[4 * j + 2] = to_acc(v.z);
-          ag[4 * j + 3] = to_acc(v.w);
-        } else {
-          fr[j - NA] = v;
-        }
+      const int xj = 8 * (Bb + j);
+      if constexpr (j + 2 < NT) rr_load_raw(lr, rrec, R[j & 1], xj + 16, d);
+      if constexpr (j + 1 < NT) rr_score_raw(R[(j + 1) & 1], S[(j + 1) & 1], Tb);
+      uint4 v;
+      t2 = rr_chain(S[j & 1], t2, K, v);
+      if constexpr (j < NA) {
+        ag[4 * j] = to_acc(v.x);
+        ag[4 * j + 1] = to_acc(v.y);
+        ag[4 * j + 2] = to_acc(v.z);
+        ag[4 * j + 3] = to_acc(v.w);
+      } else {
+        fr[j - NA] = v;
       }
-    });
+    };
+    // every slot in use (rows of at least NT aligned blocks): straight-line
+    // code, no branch between blocks for the pipeline's registers to meet at
+    if (j0 == 0) {
+      static_for<NT>([&](auto J) __attribute__((always_inline)) { slot_fwd(J); });
+    } else {
+      static_for<NT>([&](auto J) __attribute__((always_inline)) {
+        if (decltype(J)::value >= j0) slot_fwd(J);
+      });
+    }
     x = 8 * B1;
   }
   for (; x < xend; ++x) {
@@ -486,27 +493,35 @@ void k_row(BatchView b, int base, int nl) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) vr[k] = fr[NV - 1 - k];
     t2 = rr_back_blocks<NV, XB>(vr, t2, lr, B1 - 1, o, K);
-    static_for<NA / 4>([&](auto C) __attribute__((always_inline)) {
+    auto chunk_back = [&](auto C) __attribute__((always_inline)) {
       constexpr int c = NA / 4 - 1 - decltype(C)::value;   // chunks from the top down
-      if (4 * c >= j0) {
-        uint4 v[4];
-        static_for<4>([&](auto Q) __attribute__((always_inline)) {
-          constexpr int q = decltype(Q)::value, j = 4 * c + 3 - q;
-          v[q] = make_uint4(from_acc(ag[4 * j]), from_acc(ag[4 * j + 1]),
-                            from_acc(ag[4 * j + 2]), from_acc(ag[4 * j + 3]));
-        });
-        t2 = rr_back_blocks<4, XB>(v, t2, lr, Bb + 4 * c + 3, o, K);
-      } else if (4 * c + 3 >= j0) {
-        static_for<4>([&](auto Q) __attribute__((always_inline)) {
-          constexpr int j = 4 * c + 3 - decltype(Q)::value;
-          if (j >= j0) {
-            uint4 v[1] = {make_uint4(from_acc(ag[4 * j]), from_acc(ag[4 * j + 1]),
-                                     from_acc(ag[4 * j + 2]), from_acc(ag[4 * j + 3]))};
-            t2 = rr_back_blocks<1, XB>(v, t2, lr, Bb + j, o, K);
-          }
-        });
-      }
-    });
+      uint4 v[4];
+      static_for<4>([&](auto Q) __attribute__((always_inline)) {
+        constexpr int q = decltype(Q)::value, j = 4 * c + 3 - q;
+        v[q] = make_uint4(from_acc(ag[4 * j]), from_acc(ag[4 * j + 1]),
+                          from_acc(ag[4 * j + 2]), from_acc(ag[4 * j + 3]));
+      });
+      t2 = rr_back_blocks<4, XB>(v, t2, lr, Bb + 4 * c + 3, o, K);
+    };
+    if (j0 == 0) {                                       // straight-line, as in H-fwd
+      static_for<NA / 4>([&](auto C) __attribute__((always_inline)) { chunk_back(C); });
+    } else {
+      static_for<NA / 4>([&](auto C) __attribute__((always_inline)) {
+        constexpr int c = NA / 4 - 1 - decltype(C)::value;
+        if (4 * c >= j0) {
+          chunk_back(C);
+        } else if (4 * c + 3 >= j0) {
+          static_for<4>([&](auto Q) __attribute__((always_inline)) {
+            constexpr int j = 4 * c + 3 - decltype(Q)::value;
+            if (j >= j0) {
+              uint4 v[1] = {make_uint4(from_acc(ag[4 * j]), from_acc(ag[4 * j + 1]),
+                                       from_acc(ag[4 * j + 2]), from_acc(ag[4 * j + 3]))};
+              t2 = rr_back_blocks<1, XB>(v, t2, lr, Bb + j, o, K);
+            }
+          });
+        }
+      });
+    }
   }
   // blocks below the registers: LDS [max(blo, bL), Bd), then global [blo, bL)
   const int blo = min((xa + 7) >> 3, Bd);

@@ -61,9 +61,10 @@ void* de_ctx_stream(de_ctx* ctx);
  * kernels with cudaEvents inside the tool (core/tools/de_dense_cuda.cu:552-565). */
 int de_ctx_set_profiling(de_ctx* ctx, int on);
 int de_ctx_stage_ms(de_ctx* ctx, double* ms4, int* runs);
-/* Launch variants of the two scan kernels (prefetch depth; row 0..3 =
- * 2/4/8/16 blocks of 8 columns, col 0..2 = 8/16/32 rows). -1 restores the tuned/default
- * choice. Results never depend on the variant. No counterpart in the
+/* Launch variants of the two scan kernels (row 0, 1: register-resident pass
+ * with and without LDS forward slots; 2, 3: streaming pass with 8 / 16 blocks
+ * of backward prefetch. col 0..2: 8/16/32 rows in flight). -1 restores the
+ * tuned/default choice. Results never depend on the variant. No counterpart in the
  * reference; it is the knob the autotuner turns. */
 int de_ctx_set_variants(de_ctx* ctx, int row_variant, int col_variant);
 /* Message of the last failure on this thread ("" if none). */
