@@ -58,7 +58,10 @@ __device__ __forceinline__ void top2_merge(uint32_t& p0, uint32_t& p1, uint32_t 
 // those rows is unrolled, so every slot is a fixed register.
 // ACC: the register rows live in AGPRs (v_accvgpr_write / read, two rows a
 // word), which lets NRB = 5 blocks (320 rows) stay on chip at two waves a SIMD.
-template <int G, int U, bool STORE, int NRB, bool ACC = false>
+// NLR: V-fwd rows kept in the wave's LDS region (dynamic LDS, NLR * 128 bytes
+// a wave), the NLR rows just below the register rows; 0 or kWave (one V-bwd
+// block). Every frame of the launch then has H >= 64 * NRB + NLR + 2.
+template <int G, int U, bool STORE, int NRB, bool ACC = false, int NLR = 0>
 __global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(ACC ? 2 : 1)))
 void k_col(BatchView b, int base) {
   constexpr int NWL = (G + 1) / 2;           // LDS words per lane per row
@@ -70,7 +73,9 @@ void k_col(BatchView b, int base) {
   constexpr int NRR = kWave * NRB;           // rows kept in registers
   static_assert(RPW * S <= kWave && NW % S == 0, "transposed phase shape");
   static_assert(kWave % (2 * U) == 0 && 2 * U <= kVolPadRows, "prefetch depth");
+  static_assert(NLR == 0 || (NLR == kWave && NRB > 0 && !STORE), "LDS rows: one block");
   __shared__ uint32_t rb[kWave * RS];
+  extern __shared__ uint16_t lrow_all[];                 // NLR rows x 64 lanes a wave
   const int item = blockIdx.x + base;
   const int f = find_frame(b, kStageCol, item);
   const FrameDesc F = b.frames[f];
@@ -89,6 +94,8 @@ void k_col(BatchView b, int base) {
   const int slot = 2 * ((g >> 1) * kWave + lane) + (g & 1);   // u16 index within a row
   uint32_t t = uint32_t(__builtin_amdgcn_raw_buffer_load_b16(rs(0), vo, 0, 0)) << 16;
   const int yreg = H - 1 - NRR;                          // first register row
+  const int ylds = yreg - NLR;                           // first LDS row
+  uint16_t* const lrow = lrow_all + (threadIdx.x >> 6) * (NLR * kWave) + lane;
   uint32_t rr[NRB > 0 ? NRR / 2 : 1];
 
   // ---- V-fwd over rows 1 .. yend-1 in the volume, U rows in flight ----
@@ -113,7 +120,7 @@ void k_col(BatchView b, int base) {
         }
       }
     };
-    const int yend = NRB > 0 ? yreg : H;
+    const int yend = NRB > 0 ? ylds : H;
     uint32_t A[U], B[U], cA[U], cB[U];
     int y = 1;
     ldf(A, y);
@@ -136,33 +143,40 @@ void k_col(BatchView b, int base) {
     }
   }
   if constexpr (NRB > 0) {
-    // ---- V-fwd over the register rows yreg .. H-2, then row H-1 ----
+    // ---- V-fwd over the LDS rows ylds .. yreg-1 and the register rows
+    // yreg .. H-2, then row H-1 ----
+    constexpr int NC = (NLR + NRR) / U;                  // chunks
     uint32_t V[2][U], AB[2][U];
-    ldf(V[0], yreg);
-    ldc(AB[0], yreg);
+    ldf(V[0], ylds);
+    ldc(AB[0], ylds);
     uint32_t hold = 0u;                                  // odd row's half, waiting for its pair
-    static_for<NRR / U>([&](auto CC) __attribute__((always_inline)) {
+    static_for<NC>([&](auto CC) __attribute__((always_inline)) {
       constexpr int c = decltype(CC)::value;
-      const int y0 = yreg + c * U;
-      ldf(V[(c + 1) & 1], c + 1 < NRR / U ? y0 + U : H - 1);   // the last brings row H-1
-      if constexpr (c + 1 < NRR / U) ldc(AB[(c + 1) & 1], y0 + U);
+      const int y0 = ylds + c * U;
+      ldf(V[(c + 1) & 1], c + 1 < NC ? y0 + U : H - 1);  // the last brings row H-1
+      if constexpr (c + 1 < NC) ldc(AB[(c + 1) & 1], y0 + U);
       __builtin_amdgcn_sched_barrier(0);                 // the loads stay ahead of this chunk
       static_for<U>([&](auto UU) __attribute__((always_inline)) {
         constexpr int u = decltype(UU)::value;
         const uint32_t a2 = AB[c & 1][u];
         t = rf_chain(a2, t, rf_acc(a2, V[c & 1][u]));
-        constexpr int j = NRR - 1 - (c * U + u);         // row y0 + u = H-2-j; odd j first
-        if constexpr (j & 1) {
-          hold = t & 0xFFFF0000u;
+        constexpr int i = c * U + u - NLR;               // register row index
+        if constexpr (i < 0) {
+          lrow[(c * U + u) * kWave] = uint16_t(t >> 16);   // LDS row ylds + c*U + u
         } else {
-          const uint32_t w = hold | (t >> 16);
-          if constexpr (ACC) rr[j >> 1] = to_acc(w);
-          else rr[j >> 1] = w;
+          constexpr int j = NRR - 1 - i;                 // row y0 + u = H-2-j; odd j first
+          if constexpr (j & 1) {
+            hold = t & 0xFFFF0000u;
+          } else {
+            const uint32_t w = hold | (t >> 16);
+            if constexpr (ACC) rr[j >> 1] = to_acc(w);
+            else rr[j >> 1] = w;
+          }
         }
       });
     });
     const uint32_t a2 = ay[H - 1];
-    t = rf_chain(a2, t, rf_acc(a2, V[(NRR / U) & 1][0]));
+    t = rf_chain(a2, t, rf_acc(a2, V[NC & 1][0]));
     if (STORE) __builtin_amdgcn_raw_buffer_store_b16(uint16_t(t >> 16), rs(H - 1), vo, 0, 0);
   }
 
@@ -270,13 +284,18 @@ void k_col(BatchView b, int base) {
     __syncthreads();
   };
   uint32_t cA[U], cB[U];                                // coefficient chunks in flight
+  uint32_t A[U], B[U];
+  const int yb0 = H - 2 - NRR - NLR;                     // first block from the volume
   if constexpr (NRB > 0) {
-    // the register blocks: block k holds rows H-2-64k .. H-65-64k, slots 64k ..
+    // the register blocks: block k holds rows H-2-64k .. H-65-64k, slots 64k ..;
+    // then the LDS block (rows yreg-1 .. ylds) when NLR > 0
+    constexpr int NBK = NRB + (NLR > 0 ? 1 : 0);
     uint32_t CR[2][U];
     ldcb(CR[0], H - 2);
-    static_for<NRB>([&](auto KK) __attribute__((always_inline)) {
+    static_for<NBK>([&](auto KK) __attribute__((always_inline)) {
       constexpr int k = decltype(KK)::value;
       const int yb = H - 2 - kWave * k;
+      if constexpr (k == NRB) ldb(A, yb0);               // in flight over the LDS block
       static_for<kWave / U>([&](auto QQ) __attribute__((always_inline)) {
         constexpr int q = decltype(QQ)::value;
         constexpr int n = k * (kWave / U) + q;           // chunk number from the top
@@ -286,27 +305,33 @@ void k_col(BatchView b, int base) {
                                                          // one's: the volume's first)
         __builtin_amdgcn_sched_barrier(0);               // issued before this chunk's work
         const uint32_t (&ab)[U] = CR[n & 1];
-        static_for<U / 2>([&](auto PP) __attribute__((always_inline)) {
-          constexpr int u = 2 * decltype(PP)::value;
-          constexpr int j = kWave * k + q * U + u;       // even: rows j, j + 1 share a word
-          uint32_t w;
-          if constexpr (ACC) w = from_acc(rr[j >> 1]);
-          else w = rr[j >> 1];
-          bwd1(w & 0xFFFFu, ab[U - 1 - u], rr0, u, q * U + u);
-          bwd1(w >> 16, ab[U - 2 - u], rr0, u + 1, q * U + u + 1);
-        });
+        if constexpr (k < NRB) {
+          static_for<U / 2>([&](auto PP) __attribute__((always_inline)) {
+            constexpr int u = 2 * decltype(PP)::value;
+            constexpr int j = kWave * k + q * U + u;     // even: rows j, j + 1 share a word
+            uint32_t w;
+            if constexpr (ACC) w = from_acc(rr[j >> 1]);
+            else w = rr[j >> 1];
+            bwd1(w & 0xFFFFu, ab[U - 1 - u], rr0, u, q * U + u);
+            bwd1(w >> 16, ab[U - 2 - u], rr0, u + 1, q * U + u + 1);
+          });
+        } else {
+          static_for<U>([&](auto UU) __attribute__((always_inline)) {
+            constexpr int u = decltype(UU)::value;
+            constexpr int li = NLR - 1 - (q * U + u);    // row ytop - u = ylds + li
+            bwd1(uint32_t(lrow[li * kWave]), ab[U - 1 - u], rr0, u, q * U + u);
+          });
+        }
       });
       top2_block(yb, kWave);
     });
-    constexpr int nlast = NRB * (kWave / U);             // chunks done: the next is in
+    constexpr int nlast = NBK * (kWave / U);             // chunks done: the next is in
 #pragma unroll                                           // CR[nlast & 1]
     for (int i = 0; i < U; ++i) cA[i] = CR[nlast & 1][i];
   } else {
     ldcb(cA, H - 2);
   }
-  const int yb0 = H - 2 - NRR;                           // first block from the volume
-  uint32_t A[U], B[U];
-  ldb(A, yb0);
+  if constexpr (NLR == 0) ldb(A, yb0);
   for (int yb = yb0; yb >= 0; yb -= kWave) {            // block: rows yb, yb-1, ..., yb-63
     const int nrow = min(kWave, yb + 1);
     int u0 = 0;
@@ -348,14 +373,21 @@ static void launch_col_gn(const BatchView& b, int base, int items, int variant, 
 }
 
 // Register blocks by the frames' height and the launch's size. A launch of at
-// most two waves per SIMD keeps 5 blocks (320 rows) in AGPRs when every frame
-// has H >= 322 (at the default 16 rows in flight), else 3 blocks (192 rows, ~130 more VGPRs) when every
-// frame has H >= 194; a larger one 2 blocks (128 rows), which leave room for
-// more waves per SIMD; frames under 130 rows none. Measured at 848x480x128
-// (1696 waves): column pass 66 us without, 62 with 2 blocks, 60 with 3; four
-// frames in flight 0.141 / 0.137 ms a frame with 2 / 3 blocks. The 64-pair
-// batch ran 10.9 ms with 2 blocks, 11.9 with 3. Four blocks (256 VGPRs, AGPR
-// spills) took 76 us.
+// most two waves per SIMD keeps 6 register blocks (384 rows; 119 VGPRs + 128
+// AGPRs) and one LDS block (64 rows, 8 KB a wave beside the 16.6 KB top-2
+// tile of the workgroup: four workgroups of two waves a CU still fit) when
+// every frame has H >= 450 and two or more disparity groups; 5 register
+// blocks and the LDS block from H >= 386; 5 register blocks alone from
+// H >= 322 (at the default 16 rows in flight), else 3 blocks (192 rows, ~130
+// more VGPRs) when every frame has H >= 194; a larger launch 2 blocks (128
+// rows), which leave room for more waves per SIMD; frames under 130 rows none.
+// (One disparity group keeps no LDS block: a one-wave workgroup's 16.6 KB tile
+// plus 8 KB would cut the CU to six waves.) Measured at 848x480x128 (1696
+// waves): column pass 66 us without register blocks, 62 with 2, 60 with 3,
+// 53.7 with 5, 51.4 with 5 + LDS, 47.0 with 6 + LDS (31 rows left to the
+// volume); four frames in flight 0.141 / 0.137 ms a frame with 2 / 3 blocks.
+// The 64-pair batch ran 10.9 ms with 2 blocks, 11.9 with 3. Four blocks in
+// VGPRs (256 VGPRs, spills) took 76 us.
 template <int G, bool STORE>
 static void launch_col_g(const BatchView& b, int base, int items, int variant, int min_h,
                          hipStream_t s) {
@@ -366,7 +398,13 @@ static void launch_col_g(const BatchView& b, int base, int items, int variant, i
     // the volume is written back anyway (tests, the exact reverse match)
     launch_col_gn<G, STORE, 0>(b, base, items, variant, s);
   } else {
-    if (small && min_h >= 5 * kWave + 2 && variant == 1)
+    if (G >= 2 && small && min_h >= 7 * kWave + 2 && variant == 1)
+      hipLaunchKernelGGL((k_col<G, 16, STORE, 6, true, kWave>), dim3(items), dim3(64 * G),
+                         size_t(G) * kWave * kWave * 2, s, b, base);
+    else if (G >= 2 && small && min_h >= 6 * kWave + 2 && variant == 1)
+      hipLaunchKernelGGL((k_col<G, 16, STORE, 5, true, kWave>), dim3(items), dim3(64 * G),
+                         size_t(G) * kWave * kWave * 2, s, b, base);
+    else if (small && min_h >= 5 * kWave + 2 && variant == 1)
       hipLaunchKernelGGL((k_col<G, 16, STORE, 5, true>), dim3(items), dim3(64 * G), 0, s, b, base);
     else if (small && min_h >= 3 * kWave + 2) launch_col_gn<G, STORE, 3>(b, base, items, variant, s);
     else if (min_h >= 2 * kWave + 2) launch_col_gn<G, STORE, 2>(b, base, items, variant, s);
