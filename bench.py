@@ -40,7 +40,6 @@ CFG = dict(CFG7, census_hw=4, census_hh=3)                 # C2 as stated: censu
 CFG_C1 = dict(dmax=48, iters=5, sigma_s=20.0, sigma_r=0.0392, census_hw=4, census_hh=3)
 REF_FLAGS = ["--dmax", "128", "--iters", "8", "--sigma-s", "20", "--sigma-r", "0.0392"]
 HBM_PEAK_GBS = 8000.0
-ROW_STREAMING = 2          # row-pass launch variant: streaming, backward prefetch 8 blocks
 # HBM bytes per launch of each stage kernel at C2 (census 9x7), from rocprofv3
 # FETCH_SIZE (x2) and WRITE_SIZE (x1) runs on this workload, the factors read off
 # a 256 MiB device copy in the same runs: a quoted profile figure, not measured
@@ -1185,13 +1184,13 @@ def measure_c2(ctx, extra, dL, dR, cfg_kw: dict, a, W: int = W, H: int = H) -> d
     ctx.sync()
     first = disp.numpy()
     slots = []
-    # Frames in flight take the streaming row pass (about 210 registers a
-    # lane, so the row passes of two frames share a SIMD); one frame alone
-    # takes the library's choice, the register-resident pass (512 registers a
-    # lane, one wave a SIMD), which is the faster one for a lone frame.
-    # Measured at 848x480x128: resident 0.133 / 0.152 / 0.143 ms a frame with
-    # 2 / 3 / 4 in flight, streaming 0.149 / 0.132 / 0.129.
-    inflight_rv = a.row_variant if a.row_variant >= 0 else (ROW_STREAMING if nfl > 2 else -1)
+    # Frames in flight take the library's row pass (the register-resident
+    # one for an 848x480x128 frame) unless --row-variant names one. Measured
+    # at 848x480x128, ms a frame with 2 / 3 / 4 in flight: resident 0.129 /
+    # 0.131 / 0.128, resident without LDS forward slots 0.126 / 0.129 /
+    # 0.127, streaming (about 210 registers a lane, two waves a SIMD) 0.141 /
+    # 0.136 / 0.129.
+    inflight_rv = a.row_variant
     ctx.set_variants(inflight_rv, a.col_variant)
     for _ in range(nfl):
         o = (ctx.empty((H, W), np.float32), ctx.empty((H, W), np.float32))

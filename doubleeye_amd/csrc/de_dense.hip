@@ -82,13 +82,34 @@ __global__ __launch_bounds__(256) void k_prep(BatchView b) {
   const int x0 = (local % ntx) * kPrepTileW, y0 = (local / ntx) * kPrepTileH;
   const DE_CONST uint8_t* Lg = cptr(F.L);
   const DE_CONST uint8_t* Rg = cptr(F.R);
-  srf[threadIdx.x] = cptr(F.rf)[threadIdx.x];
-  for (int i = threadIdx.x; i < TWP * TH; i += 256) {
-    const int ty = i / TWP, tx = i - ty * TWP;
-    const int gx = x0 - HW + tx, gy = y0 - HH + ty;
-    const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
-    st[0][ty][tx] = in ? Lg[size_t(gy) * W + gx] : 0;
-    st[1][ty][tx] = in ? Rg[size_t(gy) * W + gx] : 0;
+  const uint16_t rfv = cptr(F.rf)[threadIdx.x];
+  {
+    // every load of the tile is issued before the first LDS store: one memory
+    // latency for the staging, not one per pass of the 256 threads
+    constexpr int NTB = TWP * TH, IT = (NTB + 255) / 256;
+    uint32_t vl[IT], vr[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const int i = threadIdx.x + 256 * k;
+      const int ty = i / TWP, tx = i - ty * TWP;
+      const int gx = x0 - HW + tx, gy = y0 - HH + ty;
+      const bool in = i < NTB && gx >= 0 && gx < W && gy >= 0 && gy < H;
+      const size_t gi = in ? size_t(gy) * W + gx : 0;    // pixel 0 stands in: loads unconditional
+      vl[k] = Lg[gi];
+      vr[k] = Rg[gi];
+      vl[k] = in ? vl[k] : 0u;
+      vr[k] = in ? vr[k] : 0u;
+    }
+    srf[threadIdx.x] = rfv;
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const int i = threadIdx.x + 256 * k;
+      if (i < NTB) {
+        const int ty = i / TWP, tx = i - ty * TWP;
+        st[0][ty][tx] = uint8_t(vl[k]);
+        st[1][ty][tx] = uint8_t(vr[k]);
+      }
+    }
   }
   __syncthreads();
   const int img = threadIdx.x >> 7;                      // wave-uniform
