@@ -40,11 +40,12 @@ CFG = dict(CFG7, census_hw=4, census_hh=3)                 # C2 as stated: censu
 CFG_C1 = dict(dmax=48, iters=5, sigma_s=20.0, sigma_r=0.0392, census_hw=4, census_hh=3)
 REF_FLAGS = ["--dmax", "128", "--iters", "8", "--sigma-s", "20", "--sigma-r", "0.0392"]
 HBM_PEAK_GBS = 8000.0
+ROW_NO_SLOTS = 1           # row-pass launch variant: resident, no LDS forward slots
 # HBM bytes per launch of each stage kernel at C2 (census 9x7), from rocprofv3
 # FETCH_SIZE (x2) and WRITE_SIZE (x1) runs on this workload, the factors read off
 # a 256 MiB device copy in the same runs: a quoted profile figure, not measured
 # in this run; the source file is named in the output line.
-TRAFFIC = {"prep": 15_534_080, "row": 122_968_064, "col": 190_605_312, "solve": 9_889_792}
+TRAFFIC = {"prep": 15_540_224, "row": 122_728_448, "col": 135_217_152, "solve": 9_889_792}
 TRAFFIC_SOURCE = "quoted: profiles/dense_c2_census9x7_848x480x128_kernels.md"
 
 
@@ -1184,13 +1185,14 @@ def measure_c2(ctx, extra, dL, dR, cfg_kw: dict, a, W: int = W, H: int = H) -> d
     ctx.sync()
     first = disp.numpy()
     slots = []
-    # Frames in flight take the library's row pass (the register-resident
-    # one for an 848x480x128 frame) unless --row-variant names one. Measured
-    # at 848x480x128, ms a frame with 2 / 3 / 4 in flight: resident 0.129 /
-    # 0.131 / 0.128, resident without LDS forward slots 0.126 / 0.129 /
-    # 0.127, streaming (about 210 registers a lane, two waves a SIMD) 0.141 /
-    # 0.136 / 0.129.
-    inflight_rv = a.row_variant
+    # Frames in flight take the register-resident row pass without LDS
+    # forward slots (its workgroups then hold little LDS, so other frames'
+    # kernels fit beside them); one frame alone takes the library's choice.
+    # Measured, ms a frame with 2 / 3 / 4 in flight at 848x480x128: resident
+    # with LDS slots 0.129 / 0.131 / 0.128, without 0.126 / 0.129 / 0.127,
+    # streaming (about 210 registers a lane, two waves a SIMD) 0.141 / 0.136 /
+    # 0.129; at 424x240x48 with four in flight 0.0358 / 0.0281 / 0.0277.
+    inflight_rv = a.row_variant if a.row_variant >= 0 else (ROW_NO_SLOTS if nfl > 1 else -1)
     ctx.set_variants(inflight_rv, a.col_variant)
     for _ in range(nfl):
         o = (ctx.empty((H, W), np.float32), ctx.empty((H, W), np.float32))
@@ -1239,8 +1241,9 @@ def measure_c2(ctx, extra, dL, dR, cfg_kw: dict, a, W: int = W, H: int = H) -> d
         o[0].free(); o[1].free()
     disp.free(); margin.free()
     return {"ms": ms, "ms_single": ms_single, "stage_ms": per, "deterministic": deterministic,
-            "inflight_row_pass": {-1: "library's choice", 0: "resident", 1: "resident",
-                                  2: "streaming", 3: "streaming"}[inflight_rv],
+            "inflight_row_pass": {-1: "library's choice", 0: "resident",
+                                  1: "resident, no LDS slots", 2: "streaming",
+                                  3: "streaming"}[inflight_rv],
             "coverage": round(float(np.isfinite(last).mean()), 4)}
 
 
