@@ -98,6 +98,26 @@ __device__ __forceinline__ void rr_score_raw(const RrRaw& R, RrBlk& S, uint32_t 
   }
 }
 
+// The same for a block of the ragged band, where lane d's window starts at
+// column lim = hw + d: the right record's index is clamped at 0 and the score
+// left of lim is kOff (a zero score).
+__device__ __forceinline__ void rr_load_raw_m(const uint4* lr, const uint4* rrec, RrRaw& R,
+                                              int x0, int d) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    R.rec[u] = lr[x0 + u];
+    R.rr[u] = *reinterpret_cast<const uint3*>(&rrec[max(x0 + u - d, 0)]);
+  }
+}
+__device__ __forceinline__ void rr_score_raw_m(const RrRaw& R, RrBlk& S, uint32_t Tb, int x0,
+                                               int lim) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    S.c[u] = x0 + u >= lim ? rr_score(R.rec[u], R.rr[u], Tb) : uint32_t(kOff);
+    S.w[u] = R.rec[u].w;
+  }
+}
+
 // Forward chain of one aligned block; its values packed two a word (hi16 of
 // consecutive steps, low then high) for the block's home. The sixteen
 // multiply-adds are one asm statement, each accumulator issued between two
@@ -377,22 +397,66 @@ void k_row(BatchView b, int base, int nl) {
   // stored: the backward pass uses kOff there) ----
   uint32_t t2 = uint32_t(kOff) << 16;
   int x = xa;
-  for (; x < xb; ++x) {                                  // ragged band: per-lane window
-    const uint4 rec = lr[x];
-    const bool ok = x >= F.hw + d;
-    const uint3 rv = *reinterpret_cast<const uint3*>(&rrec[ok ? x - d : 0]);
-    const uint32_t c = ok ? rr_score(rec, rv, Tb) : uint32_t(kOff);
-    t2 = rf_chain_z(rec.w, t2, mad_lo16(rec.w, c, K));
-    store_col(x, t2 >> 16);
-  }
-  for (; x < xm0; ++x) {
-    const uint4 rec = lr[x];
-    const uint3 rv = *reinterpret_cast<const uint3*>(&rrec[x - d]);
-    t2 = rf_chain_z(rec.w, t2, mad_lo16(rec.w, rr_score(rec, rv, Tb), K));
-    store_col(x, t2 >> 16);
-  }
+  // The ragged band [xa, xm0), where lane d's window opens at column hw + d,
+  // runs the block pipeline from xa's block with masked scores when it ends
+  // on a block boundary (every frame but the narrowest): a zero score keeps
+  // f' = kOff, so the columns of that block left of xa store kOff, as the
+  // backward scan assumes there. Column by column its latency chains cost
+  // about 10 us of the row pass at 848x480x128.
+  const int Ba = xa >> 3;
+  const bool band = (xm0 & 7) == 0 && xm0 > xa;
   RrRaw R[2];
   RrBlk S[2];
+  if (band) {
+    const int lim = F.hw + d;
+    const int Bm = xm0 >> 3;
+    auto put = [&](int bk, const uint4& v) {
+      if (bk >= bL) store_lds(bk, v);
+      else store_glb(bk, v);
+    };
+    int k = Ba;
+    if ((Bm - Ba) & 1) {                                 // one block alone
+      uint4 v;
+      rr_load_raw_m(lr, rrec, R[0], 8 * k, d);
+      rr_score_raw_m(R[0], S[0], Tb, 8 * k, lim);
+      t2 = rr_chain(S[0], t2, K, v);
+      put(k, v);
+      ++k;
+    }
+    if (k < Bm) {
+      // S[0] holds block k scored, R[1] block k+1 loaded; two blocks a step
+      rr_load_raw_m(lr, rrec, R[0], 8 * k, d);
+      rr_load_raw_m(lr, rrec, R[1], 8 * k + 8, d);
+      rr_score_raw_m(R[0], S[0], Tb, 8 * k, lim);
+      for (; k < Bm; k += 2) {
+        uint4 v;
+        rr_load_raw_m(lr, rrec, R[0], 8 * k + 16, d);
+        rr_score_raw_m(R[1], S[1], Tb, 8 * k + 8, lim);
+        t2 = rr_chain(S[0], t2, K, v);
+        put(k, v);
+        rr_load_raw_m(lr, rrec, R[1], 8 * k + 24, d);
+        rr_score_raw_m(R[0], S[0], Tb, 8 * k + 16, lim);
+        t2 = rr_chain(S[1], t2, K, v);
+        put(k + 1, v);
+      }
+    }
+    x = xm0;
+  } else {
+    for (; x < xb; ++x) {                                // ragged band: per-lane window
+      const uint4 rec = lr[x];
+      const bool ok = x >= F.hw + d;
+      const uint3 rv = *reinterpret_cast<const uint3*>(&rrec[ok ? x - d : 0]);
+      const uint32_t c = ok ? rr_score(rec, rv, Tb) : uint32_t(kOff);
+      t2 = rf_chain_z(rec.w, t2, mad_lo16(rec.w, c, K));
+      store_col(x, t2 >> 16);
+    }
+    for (; x < xm0; ++x) {
+      const uint4 rec = lr[x];
+      const uint3 rv = *reinterpret_cast<const uint3*>(&rrec[x - d]);
+      t2 = rf_chain_z(rec.w, t2, mad_lo16(rec.w, rr_score(rec, rv, Tb), K));
+      store_col(x, t2 >> 16);
+    }
+  }
   if (npre & 1) {                                        // block B0 alone
     uint4 v;
     rr_load_raw(lr, rrec, R[0], x, d);
@@ -524,14 +588,21 @@ void k_row(BatchView b, int base, int nl) {
     }
   }
   // blocks below the registers: LDS [max(blo, bL), Bd), then global [blo, bL)
-  const int blo = min((xa + 7) >> 3, Bd);
+  const int blo = band ? Ba : min((xa + 7) >> 3, Bd);
   t2 = rr_back_range<4, XB>(
       [&](int bk) { return *reinterpret_cast<const uint4*>(lds_block(bk)); }, t2, lr, Bd - 1,
       max(blo, bL), o, K);
   t2 = rr_back_range<P, XB>(
       [&](int bk) { return *reinterpret_cast<const DE_GLOBAL uint4*>(glb_block(bk)); }, t2, lr,
       min(bL, Bd) - 1, blo, o, K);
-  for (x = 8 * blo - 1; x >= 0; --x) back1(x);
+  if (band) {
+    // blocks left of xa's: every forward value is kOff
+    const uint32_t ko = uint32_t(kOff) * 0x10001u;
+    t2 = rr_back_range<4, XB>([&](int) { return make_uint4(ko, ko, ko, ko); }, t2, lr, Ba - 1, 0,
+                              o, K);
+  } else {
+    for (x = 8 * blo - 1; x >= 0; --x) back1(x);
+  }
 }
 
 // ---- the streaming row pass (k_rows) ----
