@@ -98,6 +98,17 @@ __device__ __forceinline__ void rr_score_raw(const RrRaw& R, RrBlk& S, uint32_t 
   }
 }
 
+// The same for a block at the row's end: columns from xe on (the right
+// border, and the padding past W) score kOff.
+__device__ __forceinline__ void rr_score_raw_e(const RrRaw& R, RrBlk& S, uint32_t Tb, int x0,
+                                               int xe) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    S.c[u] = x0 + u < xe ? rr_score(R.rec[u], R.rr[u], Tb) : uint32_t(kOff);
+    S.w[u] = R.rec[u].w;
+  }
+}
+
 // The same for a block of the ragged band, where lane d's window starts at
 // column lim = hw + d: the right record's index is clamped at 0 and the score
 // left of lim is kOff (a zero score).
@@ -531,16 +542,31 @@ void k_row(BatchView b, int base, int nl) {
     }
     x = 8 * B1;
   }
-  for (; x < xend; ++x) {
-    const uint4 rec = lr[x];
-    const uint3 rv = *reinterpret_cast<const uint3*>(&rrec[x - d]);
-    t2 = rf_chain_z(rec.w, t2, mad_lo16(rec.w, rr_score(rec, rv, Tb), K));
-    store_col(x, t2 >> 16);
-  }
-  for (; x < W; ++x) {                                   // right border: zero scores
-    const uint32_t w = lr[x].w;
-    t2 = rf_chain_z(w, t2, mad_lo16(w, uint32_t(kOff), K));
-    store_col(x, t2 >> 16);
+  if (x == 8 * B1) {
+    // the columns from the aligned range's end to W as whole blocks in the
+    // tail slots (at most two: xend >= W - 7 and the range ends at xend & ~7),
+    // zero scores from xend on; a block may run past W into the records'
+    // padding, so the state after column W-1 is its stored value
+    for (; x < W; x += 8) {
+      uint4 v;
+      rr_load_raw_m(lr, rrec, R[0], x, d);                 // (narrow rows: x - d may be < 0)
+      rr_score_raw_e(R[0], S[0], Tb, x, xend);
+      t2 = rr_chain(S[0], t2, K, v);
+      store_lds(x >> 3, v);
+    }
+    t2 = load_col(W - 1) << 16;
+  } else {
+    for (; x < xend; ++x) {
+      const uint4 rec = lr[x];
+      const uint3 rv = *reinterpret_cast<const uint3*>(&rrec[x - d]);
+      t2 = rf_chain_z(rec.w, t2, mad_lo16(rec.w, rr_score(rec, rv, Tb), K));
+      store_col(x, t2 >> 16);
+    }
+    for (; x < W; ++x) {                                 // right border: zero scores
+      const uint32_t w = lr[x].w;
+      t2 = rf_chain_z(w, t2, mad_lo16(w, uint32_t(kOff), K));
+      store_col(x, t2 >> 16);
+    }
   }
 
   // ---- H-bwd: g'[W-1] = f'[W-1]; coefficient of x+1 for column x ----
