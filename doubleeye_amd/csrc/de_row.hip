@@ -651,6 +651,27 @@ __device__ __forceinline__ void rows_load_raw(const RowLds& q, RowRaw& R, int x0
   }
 }
 
+// The ragged band's form of the two above: right index clamped at 0, score
+// kOff left of lane d's window (column lim = hw + d).
+__device__ __forceinline__ void rows_load_raw_m(const RowLds& q, RowRaw& R, int x0, int d) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int j = max(x0 + u - d, 0);
+    R.rec[u] = q.lr[x0 + u];
+    R.cr[u] = q.crs[j];
+    R.r2[u] = q.r2s[j];
+  }
+}
+__device__ __forceinline__ void row_score_raw_m(const RowLds& q, const RowRaw& R, RowBlk& S,
+                                                int x0, int lim) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    S.c[u] = x0 + u >= lim ? row_score(q, R.rec[u], R.cr[u], R.r2[u]) : uint32_t(kOff);
+    S.z[u] = R.rec[u].z;
+    S.w[u] = R.rec[u].w;
+  }
+}
+
 // Chain of one aligned 8-column block; its forward values leave as one 16-byte
 // word per lane (hi16 of consecutive steps packed low, high).
 __device__ __forceinline__ uint32_t rows_chain_fwd(const RowBlk& S, uint32_t t2,
@@ -807,15 +828,46 @@ __global__ __launch_bounds__(256) void k_rows(BatchView b, int base, int items, 
   // ---- H-fwd. x in [0, xa): every score is zero, so f' stays kOff (not
   // stored: the backward pass uses kOff there) ----
   uint32_t t2 = uint32_t(kOff) << 16;
-  for (int x = xa; x < xb; ++x) {                        // ragged band: per-lane window
-    const uint4 rec = lr[x];
-    const bool ok = x >= F.hw + d;
-    const int j = ok ? x - d : 0;
-    const uint32_t c = ok ? row_score(q, rec, crs[j], r2s[j]) : uint32_t(kOff);
-    t2 = rf_chain_z(rec.z, t2, __umul24(rec.w, c) + 32768u);
-    *fwd_at(x) = uint16_t(t2 >> 16);
-  }
   const int xm0 = min((xb + 7) & ~7, xend);              // aligned blocks [xm0, xm0 + 8 nblk)
+  // the ragged band [xa, xm0) as masked blocks from xa's block when it ends on
+  // a block boundary (as in k_row: the block's columns left of xa store kOff)
+  const int Ba = xa >> 3;
+  const bool band = (xm0 & 7) == 0 && xm0 > xa;
+  if (band) {
+    const int lim = F.hw + d;
+    const int Bm = xm0 >> 3;
+    RowRaw R0, R1;
+    RowBlk S0, S1;
+    int k = Ba;
+    if ((Bm - Ba) & 1) {                                 // one block alone
+      rows_load_raw_m(q, R0, 8 * k, d);
+      row_score_raw_m(q, R0, S0, 8 * k, lim);
+      t2 = rows_chain_fwd(S0, t2, o.frow + size_t(k) * 1024 + fo);
+      ++k;
+    }
+    if (k < Bm) {
+      rows_load_raw_m(q, R0, 8 * k, d);
+      rows_load_raw_m(q, R1, 8 * k + 8, d);
+      row_score_raw_m(q, R0, S0, 8 * k, lim);
+      for (; k < Bm; k += 2) {
+        rows_load_raw_m(q, R0, 8 * k + 16, d);
+        row_score_raw_m(q, R1, S1, 8 * k + 8, lim);
+        t2 = rows_chain_fwd(S0, t2, o.frow + size_t(k) * 1024 + fo);
+        rows_load_raw_m(q, R1, 8 * k + 24, d);
+        row_score_raw_m(q, R0, S0, 8 * k + 16, lim);
+        t2 = rows_chain_fwd(S1, t2, o.frow + size_t(k + 1) * 1024 + fo);
+      }
+    }
+  } else {
+    for (int x = xa; x < xb; ++x) {                      // ragged band: per-lane window
+      const uint4 rec = lr[x];
+      const bool ok = x >= F.hw + d;
+      const int j = ok ? x - d : 0;
+      const uint32_t c = ok ? row_score(q, rec, crs[j], r2s[j]) : uint32_t(kOff);
+      t2 = rf_chain_z(rec.z, t2, __umul24(rec.w, c) + 32768u);
+      *fwd_at(x) = uint16_t(t2 >> 16);
+    }
+  }
   const int nblk_all = max(0, ((xend & ~7) - xm0) >> 3);
   // the last kRowSRegBlocks aligned blocks [E - NB, E) go to registers when at
   // least two blocks precede them
@@ -824,7 +876,7 @@ __global__ __launch_bounds__(256) void k_rows(BatchView b, int base, int items, 
   const int nblk = use_reg ? nblk_all - NB : nblk_all;   // blocks through the scratch
   const int E = (xm0 >> 3) + nblk_all;
   uint4 fr[NB];
-  int x = xb;
+  int x = band ? xm0 : xb;
   for (; x < xm0; ++x) {
     const uint4 rec = lr[x];
     t2 = rf_chain_z(rec.z, t2,
@@ -908,7 +960,7 @@ __global__ __launch_bounds__(256) void k_rows(BatchView b, int base, int items, 
     if ((xc & 7) == 0) row_flush<XB, true>(o, xc >> 3);
   };
   const int bhi = (W - 1) >> 3;
-  const int blo = min(bhi, (xa + 7) >> 3);
+  const int blo = min(bhi, band ? Ba : (xa + 7) >> 3);
   o.tile[((W - 1) & 7) * kWave + lane] = uint16_t(t2 >> 16);
   if (((W - 1) & 7) == 0) row_flush<XB, true>(o, bhi);
   for (x = W - 2; x >= 8 * bhi; --x) back1(x);
@@ -952,7 +1004,16 @@ __global__ __launch_bounds__(256) void k_rows(BatchView b, int base, int items, 
     row_load_blocks<1>(v, o, bb);
     t2 = rows_back_blocks<1, XB>(v, t2, lr, bb, o);
   }
-  for (x = 8 * blo - 1; x >= 0; --x) back1(x);
+  if (band) {
+    // blocks left of xa's: every forward value is kOff
+    const uint32_t ko = uint32_t(kOff) * 0x10001u;
+    for (int bk = Ba - 1; bk >= 0; --bk) {
+      uint4 v[1] = {make_uint4(ko, ko, ko, ko)};
+      t2 = rows_back_blocks<1, XB>(v, t2, lr, bk, o);
+    }
+  } else {
+    for (x = 8 * blo - 1; x >= 0; --x) back1(x);
+  }
 }
 
 // ---------------------------------------------------------------------------
