@@ -228,18 +228,28 @@ void k_col(BatchView b, int base) {
     if (cand) {
       const uint32_t* row = rb + r * RS + sp * WPS;
       if (kfull) {
+        // keys built by one byte permute each: 1 << 24 | v' << 8 | 255 - k
+        // (the same order as the packed keys; the top byte marks a key, so no
+        // key is 0), then turned into the packed form. Word j holds k = klo
+        // and klo + 64, klo = ((j >> 6) << 7) + (j & 63), consecutive within a
+        // lane's share: the constant word cw (255 - klo in byte 0, 191 - klo
+        // in byte 1) steps down by 0x101 a word.
+        const int j0 = sp * WPS;
+        const int klo0 = ((j0 >> 6) << 7) + (j0 & 63);
+        uint32_t cw = 0x01000000u | uint32_t(191 - klo0) << 8 | uint32_t(255 - klo0);
 #pragma unroll 8
         for (int jj = 0; jj < WPS; ++jj) {
-          const int j = sp * WPS + jj;
           const uint32_t wv = row[jj];
-          const int klo = ((j >> 6) << 7) + (j & 63);   // word j holds k = klo and klo + 64
-          const uint32_t a = (wv << 8 & 0xFFFF00u) + uint32_t(256 + 255 - klo);
-          const uint32_t c = (wv >> 8 & 0xFFFF00u) + uint32_t(256 + 255 - 64 - klo);
+          const uint32_t a = __builtin_amdgcn_perm(cw, wv, 0x07010004u);   // cw.b0 wv.b0 wv.b1 cw.b3
+          const uint32_t c = __builtin_amdgcn_perm(cw, wv, 0x07030205u);   // cw.b1 wv.b2 wv.b3 cw.b3
+          cw -= 0x101u;
           p1 = med3(p0, p1, a);
           p0 = max(p0, a);
           p1 = med3(p0, p1, c);
           p0 = max(p0, c);
         }
+        p0 -= 0xFFFF00u;                                 // to (v' + 1) << 8 | 255 - k
+        p1 -= 0xFFFF00u;
       } else {
         // left-border columns (kmax below the group range): bounds per key
 #pragma unroll 2
