@@ -45,7 +45,7 @@ ROW_NO_SLOTS = 1           # row-pass launch variant: resident, no LDS forward s
 # FETCH_SIZE (x2) and WRITE_SIZE (x1) runs on this workload, the factors read off
 # a 256 MiB device copy in the same runs: a quoted profile figure, not measured
 # in this run; the source file is named in the output line.
-TRAFFIC = {"prep": 15_540_224, "row": 122_728_448, "col": 135_217_152, "solve": 9_889_792}
+TRAFFIC = {"prep": 15_540_224, "row": 122_910_720, "col": 135_194_624, "solve": 9_889_792}
 TRAFFIC_SOURCE = "quoted: profiles/dense_c2_census9x7_848x480x128_kernels.md"
 
 
