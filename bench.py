@@ -1189,9 +1189,9 @@ def measure_c2(ctx, extra, dL, dR, cfg_kw: dict, a, W: int = W, H: int = H) -> d
     # forward slots (its workgroups then hold little LDS, so other frames'
     # kernels fit beside them); one frame alone takes the library's choice.
     # Measured, ms a frame with 2 / 3 / 4 in flight at 848x480x128: resident
-    # with LDS slots 0.129 / 0.131 / 0.128, without 0.126 / 0.129 / 0.127,
-    # streaming (about 210 registers a lane, two waves a SIMD) 0.141 / 0.136 /
-    # 0.129; at 424x240x48 with four in flight 0.0358 / 0.0281 / 0.0277.
+    # with LDS slots 0.123 / 0.120 / 0.119, without 0.118 / 0.122 / 0.119,
+    # streaming (about 210 registers a lane, two waves a SIMD) 0.135 / 0.127 /
+    # 0.123; at 424x240x48 with four in flight 0.0358 / 0.0281 / 0.0277.
     inflight_rv = a.row_variant if a.row_variant >= 0 else (ROW_NO_SLOTS if nfl > 1 else -1)
     ctx.set_variants(inflight_rv, a.col_variant)
     for _ in range(nfl):
