@@ -50,11 +50,15 @@ namespace de {
 //
 // The forward scan is software-pipelined over 8-column blocks in three
 // stages (LDS loads of block n+2, score-table reads of block n+1, chain of
-// block n). The backward scan's final values leave as one 2-byte store a lane
-// a column (global_store_short_d16_hi of the chain word), a wave's store
-// covering the 128 contiguous bytes of its disparity group in the [y][x][Dpad]
-// volume: no LDS round trip stands between the chain and the store (an LDS
-// transpose into 16-byte stores took 71.7 us against 69.2 us at 848x480x128).
+// block n), from the block of the wave's first valid column to the row's end:
+// the ragged band (lane d's window opens at column hw + d) and the columns
+// past the aligned range take masked scores (kOff, a zero score, outside the
+// window), so no column runs alone. The backward scan's final values leave
+// as one 2-byte store a lane a column (global_store_short_d16_hi of the chain
+// word), a wave's store covering the 128 contiguous bytes of its disparity
+// group in the [y][x][Dpad] volume: no LDS round trip stands between the chain
+// and the store (an LDS transpose into 16-byte stores took 71.7 us against
+// 69.2 us at 848x480x128).
 
 constexpr int kRowVBlocks = 16;      // forward blocks in VGPRs (64 registers)
 constexpr int kRowABlocks = 60;      // forward blocks in AGPRs (240 registers)
