@@ -3,7 +3,8 @@
 //
 // Three launches per batch:
 //   k_sp_sort   one workgroup per frame: bitonic sort of the right keypoints by
-//               (y, index) in LDS, the epipolar-band index (match.cpp:52-57).
+//               (y, index) in registers and LDS, the epipolar-band index
+//               (match.cpp:52-57).
 //   k_sp_cand   one wave per left keypoint: 64-ary search of the band, scores
 //               in lanes, the kept candidates (match.cpp:61-98, 295-332).
 //   k_sp_solve  one 1024-thread workgroup per frame: column lists, damped rho /
@@ -219,11 +220,71 @@ __device__ void put_match(const SpDesc& f, int slot, int e, float belief) {
 // ---------------------------------------------------------------------------
 // k_sp_sort: right keypoints by (y, index).
 
+// Up to 2 * kSpThreads keys (every frame of the bench and the tests) the
+// network runs in registers: thread t holds positions t and t + kSpThreads,
+// partners closer than a wave come by lane shuffles, the pair kSpThreads apart
+// is within the thread, and only the distances 64 .. kSpThreads / 2 exchange
+// through LDS (two alternating buffers, one barrier a step): 14 barriers at
+// 2048 keys instead of 66.
+__device__ __forceinline__ void sort_keys_reg(const SpDesc& f, int P, uint64_t* keys) {
+  const int t0 = threadIdx.x, t1 = t0 + kSpThreads;
+  auto key = [&](int t) -> uint64_t {
+    return t < f.nr ? (uint64_t(ordered(f.R[t].y)) << 32) | uint32_t(t) : ~0ull;
+  };
+  uint64_t a[2] = {key(t0), key(t1)};
+  const int tp[2] = {t0, t1};
+  int pb = 0;
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j == kSpThreads) {                             // positions t0 < t1 = t0 ^ j
+        const bool asc = (t0 & k) == 0;
+        const uint64_t lo = a[0] < a[1] ? a[0] : a[1], hi = a[0] < a[1] ? a[1] : a[0];
+        a[0] = asc ? lo : hi;
+        a[1] = asc ? hi : lo;
+        continue;
+      }
+      uint64_t b[2];
+      if (j < kWave) {
+        b[0] = __shfl_xor(a[0], j);
+        b[1] = __shfl_xor(a[1], j);
+      } else {
+        uint64_t* buf = keys + pb * (2 * kSpThreads);
+        buf[t0] = a[0];
+        buf[t1] = a[1];
+        __syncthreads();
+        b[0] = buf[t0 ^ j];
+        b[1] = buf[t1 ^ j];
+        pb ^= 1;
+      }
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        // ascending blocks put the smaller key at the lower position
+        const bool take_min = ((tp[e] & j) == 0) == ((tp[e] & k) == 0);
+        const bool lt = a[e] < b[e];
+        a[e] = take_min ? (lt ? a[e] : b[e]) : (lt ? b[e] : a[e]);
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int t = tp[e];
+    if (t < f.nr) {
+      const int jj = int(uint32_t(a[e]));
+      f.yj[t] = jj;
+      f.ys[t] = f.R[jj].y;
+    }
+  }
+}
+
 __global__ __launch_bounds__(kSpThreads) void k_sp_sort(const SpDesc* __restrict__ frames) {
   extern __shared__ uint64_t keys[];
   const SpDesc f = frames[blockIdx.x];
   int P = 1;
   while (P < f.nr) P <<= 1;
+  if (P <= 2 * kSpThreads) {
+    sort_keys_reg(f, P, keys);
+    return;
+  }
   for (int t = threadIdx.x; t < P; t += kSpThreads)
     keys[t] = t < f.nr ? (uint64_t(ordered(f.R[t].y)) << 32) | uint32_t(t) : ~0ull;
   __syncthreads();
@@ -490,16 +551,21 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
     f.rt[j] = 0;
   }
   __syncthreads();
+  // A left keypoint's edges (i*K .. i*K+n-1 of ej and es) are read kChunk at a
+  // time, every load of a group issued before the first use: one memory
+  // latency a group, not an edge (most rows have at most kChunk edges).
   int mine = 0;
   for (int i = tid; i < nl; i += kSpThreads) {
     const int n = f.rc[i];
     mine += n;
     f.lt[i] = 0;
-    for (int k = 0; k < n; ++k) {
-      const int e = i * K + k;
-      atomicAdd(&col_end[f.ej[e]], 1);
-      f.rho[e] = 0.f;
-      f.beta[e] = 0.f;
+    for (int k0 = 0; k0 < n; k0 += kChunk) {
+      int jv[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) jv[u] = k0 + u < n ? f.ej[i * K + k0 + u] : 0;
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u)
+        if (k0 + u < n) atomicAdd(&col_end[jv[u]], 1);
     }
   }
   int E = 0;
@@ -522,9 +588,13 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
   __syncthreads();
   for (int i = tid; i < nl; i += kSpThreads) {
     const int n = f.rc[i];
-    for (int k = 0; k < n; ++k) {
-      const int e = i * K + k;
-      f.col_edges[atomicAdd(&col_end[f.ej[e]], 1)] = e;
+    for (int k0 = 0; k0 < n; k0 += kChunk) {
+      int jv[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) jv[u] = k0 + u < n ? f.ej[i * K + k0 + u] : 0;
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u)
+        if (k0 + u < n) f.col_edges[atomicAdd(&col_end[jv[u]], 1)] = i * K + k0 + u;
     }
   }
   __syncthreads();
@@ -534,10 +604,17 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
   for (int i = tid; i < nl; i += kSpThreads) {
     const int n = f.rc[i];
     float t = kNeg, r = kNeg;
-    for (int k = 0; k < n; ++k) {
-      const float s = f.es[i * K + k];
-      if (s > t) { r = t; t = s; }
-      else if (s > r) r = s;
+    for (int k0 = 0; k0 < n; k0 += kChunk) {
+      float sv[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) sv[u] = k0 + u < n ? f.es[i * K + k0 + u] : kNeg;
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        if (k0 + u >= n) break;
+        const float s = sv[u];
+        if (s > t) { r = t; t = s; }
+        else if (s > r) r = s;
+      }
     }
     float m = 0.f;
     if (t != kNeg) m = t - (r == kNeg ? f.lambda : fmax_ref(r, f.lambda));
@@ -635,16 +712,34 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
     __syncthreads();
     for (int i = tid; i < nl; i += kSpThreads) {
       const int c0 = rowst[i], n = rowst[i + 1] - c0;
-      for (int k = 0; k < n; ++k) {
-        rho_s[c0 + k] = 0.f;
-        beta_s[c0 + k] = 0.f;
-        es_s[c0 + k] = f.es[i * K + k];
+      for (int k0 = 0; k0 < n; k0 += kChunk) {
+        float sv[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) sv[u] = k0 + u < n ? f.es[i * K + k0 + u] : 0.f;
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          if (k0 + u >= n) break;
+          rho_s[c0 + k0 + u] = 0.f;
+          beta_s[c0 + k0 + u] = 0.f;
+          es_s[c0 + k0 + u] = sv[u];
+        }
       }
     }
-    for (int q = tid; q < E; q += kSpThreads) {
-      const int e = f.col_edges[q];
-      const int i = e / K;
-      colq[q] = uint16_t(rowst[i] + (e - i * K));
+    for (int q0 = tid; q0 < E; q0 += kChunk * kSpThreads) {
+      int ev[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const int q = q0 + u * kSpThreads;
+        ev[u] = q < E ? f.col_edges[q] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const int q = q0 + u * kSpThreads;
+        if (q < E) {
+          const int i = ev[u] / K;
+          colq[q] = uint16_t(rowst[i] + (ev[u] - i * K));
+        }
+      }
     }
     __syncthreads();
     for (int it = 0; it < f.iterations; ++it) {
@@ -736,6 +831,14 @@ __global__ __launch_bounds__(kSpThreads) void k_sp_solve(const SpDesc* __restric
     __syncthreads();
     // col_end's region was not touched; the decision below reads it
   } else {
+    for (int i = tid; i < nl; i += kSpThreads) {
+      const int n = f.rc[i];
+      for (int k = 0; k < n; ++k) {
+        f.rho[i * K + k] = 0.f;
+        f.beta[i * K + k] = 0.f;
+      }
+    }
+    __syncthreads();
     for (int it = 0; it < f.iterations; ++it) {
       float delta = 0.f;
       // rho_ij = s_ij - max(lambda, max_{k != j} beta_ik): competition in a row.
@@ -1019,7 +1122,7 @@ void sp_fill_cfg(const de_match_cfg& c, int mode, float ratio, SpDesc* d) {
 size_t sort_lds_bytes(int nr) {
   int P = 1;
   while (P < nr) P <<= 1;
-  return size_t(P) * 8;
+  return P <= 2 * kSpThreads ? size_t(2) * (2 * kSpThreads) * 8 : size_t(P) * 8;
 }
 
 size_t max_sort_lds(const std::vector<SpDesc>& fr) {
