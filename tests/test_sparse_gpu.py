@@ -162,3 +162,22 @@ def test_invalid_arguments_raise(ctx):
         sparse.match_masda(L, [0], R, np.zeros(9000, np.uint64), ctx=ctx)
     with pytest.raises(DeError):
         sparse.match_masda(L, [0], L, [0], gcfg(dict(sc.DEFAULTS, iterations=-1)), ctx=ctx)
+
+
+@pytest.mark.parametrize("nr", [1024, 1025, 2048, 2049, 2600])
+def test_sort_sizes_equal_oracle(ctx, nr):
+    """Right-keypoint counts on both sides of the sort's forms: up to 2048 keys
+    in registers (1024 and 2048 fill the network exactly, 1025 just passes a
+    power of two), past that the LDS network."""
+    from doubleeye_amd import sparse, synth
+    from oracle import sparse_binding as sb
+    lxy, dl, rxy, dr, _ = synth.sparse_problem(nr, 848, 480, 40 + nr, outliers=0.25)
+    assert len(rxy) >= nr
+    rxy, dr = np.ascontiguousarray(rxy[:nr]), np.ascontiguousarray(dr[:nr])
+    note(f"[sparse sort] {len(lxy)}x{len(rxy)}")
+    c = dict(sc.DEFAULTS, max_candidates=8, iterations=6)
+    st = sparse.MatchStats()
+    m = sparse.match_masda(lxy, dl, rxy, dr, gcfg(c), st, ctx=ctx)
+    om, ost = sb.match_masda(lxy, dl, rxy, dr, ocfg(c), canonical=True)
+    assert sc.same_matches(m, om)
+    assert sc.stats_tuple(st) == sc.stats_tuple(ost)
