@@ -57,7 +57,9 @@ __device__ __forceinline__ void top2_merge(uint32_t& p0, uint32_t& p1, uint32_t 
 // back. Row H-2-j sits in half j & 1 of register word j >> 1; the code over
 // those rows is unrolled, so every slot is a fixed register.
 // ACC: the register rows live in AGPRs (v_accvgpr_write / read, two rows a
-// word), which lets NRB = 5 blocks (320 rows) stay on chip at two waves a SIMD.
+// word; the 128 AGPRs a wave gets at two waves a SIMD hold 256 rows, the rest
+// sit in VGPRs), which lets NRB = 5 or 6 blocks (320 or 384 rows) stay on
+// chip at two waves a SIMD.
 // NLR: V-fwd rows kept in the wave's LDS region (dynamic LDS, NLR * 128 bytes
 // a wave), the NLR rows just below the register rows; 0 or kWave (one V-bwd
 // block). Every frame of the launch then has H >= 64 * NRB + NLR + 2.
