@@ -136,6 +136,17 @@ def test_c1_census_9x7(ctx, oracle):
     assert np.isfinite(d).mean() > 0.5
 
 
+@pytest.mark.parametrize("w,h,d", [(424, 560, 128), (1280, 720, 192)])
+def test_multi_round_row_pass(ctx, oracle, w, h, d):
+    """More row waves than SIMDs (560 x 2 and 720 x 3 against 1,024): the
+    register-resident row pass runs in rounds, without LDS forward slots; the
+    first frame's column pass keeps six register blocks and the LDS block."""
+    from doubleeye_amd import synth
+    L, R, _ = synth.slanted_pair(w, h, seed=5)
+    kw = dict(dmax=d, iters=4, sigma_s=20.0, sigma_r=0.0392, census_hw=4, census_hh=3)
+    check_dense(ctx, oracle, L, R, kw, f"multi-round {w}x{h}x{d}")
+
+
 def test_9x7_extension_and_dmin0(ctx, oracle):
     from doubleeye_amd import synth
     L, R, _ = synth.fronto_pair(300, 160, seed=9)
