@@ -94,11 +94,19 @@ def test_plan_graph_replays(ctx):
         plan.close()
 
 
-def test_variants_do_not_change_results(ctx):
+@pytest.mark.parametrize("w,h,dmax,win,dhi", [(424, 240, 96, (3, 3), 80.0),
+                                              (848, 480, 128, (4, 3), 110.0)])
+def test_variants_do_not_change_results(ctx, w, h, dmax, win, dhi):
+    """Every row-pass and column-pass variant gives the default's planes. At
+    848x480 the row waves fit one round: variant 0 keeps forward blocks in LDS
+    slots, variant 1 (the frames-in-flight choice) sends them through the
+    global scratch, 2 and 3 stream the whole forward row."""
     from doubleeye_amd import synth
     from doubleeye_amd.dense import Cfg
-    L, R, _ = synth.slanted_pair(424, 240, seed=7, dhi=80.0)
-    cfg = Cfg(dmax=96, iters=4, sigma_s=20.0, sigma_r=0.0392)
+    L, R, _ = synth.slanted_pair(w, h, seed=7, dhi=dhi)
+    cfg = Cfg(dmax=dmax, iters=4, sigma_s=20.0, sigma_r=0.0392, census_hw=win[0],
+              census_hh=win[1])
+    note(f"[variants] {w}x{h} D={dmax} census {2 * win[0] + 1}x{2 * win[1] + 1}")
     ref = ctx.run_dense(L, R, cfg, want_cand=True)
     try:
         for rv in range(4):
@@ -108,8 +116,8 @@ def test_variants_do_not_change_results(ctx):
                 assert all(same(a, b) for a, b in zip(got, ref)), (rv, cv)
     finally:
         ctx.set_variants(-1, -1)
-    note("[autotune] 424x240 D=96")
-    rv, cv = ctx.autotune(424, 240, cfg, reps=3)
+    note(f"[autotune] {w}x{h} D={dmax}")
+    rv, cv = ctx.autotune(w, h, cfg, reps=3)
     assert 0 <= rv < 4 and 0 <= cv < 3
     assert all(same(a, b) for a, b in zip(ctx.run_dense(L, R, cfg, want_cand=True), ref))
 
