@@ -133,6 +133,21 @@ __device__ __forceinline__ void rr_score_raw_m(const RrRaw& R, RrBlk& S, uint32_
   }
 }
 
+// Block n's LDS operands loaded column by column between the scoring steps of
+// block c (loaded one step earlier), so the wave's 16 operand reads spread
+// over the block's VALU work instead of leaving in one burst ahead of it.
+__device__ __forceinline__ void rr_load_score(const uint4* lr, const uint4* rrec, RrRaw& Rn,
+                                              int x0n, int d, const RrRaw& Rc, RrBlk& S,
+                                              uint32_t Tb) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    Rn.rec[u] = lr[x0n + u];
+    Rn.rr[u] = *reinterpret_cast<const uint3*>(&rrec[x0n + u - d]);
+    S.c[u] = rr_score(Rc.rec[u], Rc.rr[u], Tb);
+    S.w[u] = Rc.rec[u].w;
+  }
+}
+
 // Forward chain of one aligned block; its values packed two a word (hi16 of
 // consecutive steps, low then high) for the block's home. The sixteen
 // multiply-adds are one asm statement, each accumulator issued between two
@@ -489,12 +504,10 @@ void k_row(BatchView b, int base, int nl) {
     rr_score_raw(R[0], S[0], Tb);
     auto pair = [&](int k, auto&& put) {
       uint4 v;
-      rr_load_raw(lr, rrec, R[0], 8 * k + 16, d);
-      rr_score_raw(R[1], S[1], Tb);
+      rr_load_score(lr, rrec, R[0], 8 * k + 16, d, R[1], S[1], Tb);
       t2 = rr_chain(S[0], t2, K, v);
       put(k, v);
-      rr_load_raw(lr, rrec, R[1], 8 * k + 24, d);
-      rr_score_raw(R[0], S[0], Tb);
+      rr_load_score(lr, rrec, R[1], 8 * k + 24, d, R[0], S[0], Tb);
       t2 = rr_chain(S[1], t2, K, v);
       put(k + 1, v);
     };
@@ -522,8 +535,11 @@ void k_row(BatchView b, int base, int nl) {
     auto slot_fwd = [&](auto J) __attribute__((always_inline)) {
       constexpr int j = decltype(J)::value;
       const int xj = 8 * (Bb + j);
-      if constexpr (j + 2 < NT) rr_load_raw(lr, rrec, R[j & 1], xj + 16, d);
-      if constexpr (j + 1 < NT) rr_score_raw(R[(j + 1) & 1], S[(j + 1) & 1], Tb);
+      if constexpr (j + 2 < NT) {
+        rr_load_score(lr, rrec, R[j & 1], xj + 16, d, R[(j + 1) & 1], S[(j + 1) & 1], Tb);
+      } else if constexpr (j + 1 < NT) {
+        rr_score_raw(R[(j + 1) & 1], S[(j + 1) & 1], Tb);
+      }
       uint4 v;
       t2 = rr_chain(S[j & 1], t2, K, v);
       if constexpr (j < NA) {
